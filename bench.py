@@ -1,0 +1,246 @@
+"""Benchmark of the MI355X neighbour-list engine (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Headline workload (one "step"): the 256-system ragged batch (N log-uniform in
+[8, 2048], half isolated, half periodic orthorhombic/triclinic at liquid
+density) built in ONE pass with three lists at cutoffs {5, 6, 14} A, inputs
+resident in HBM, outputs left on the device.  `value` = edges/s summed over
+the three lists.  Beside it: systems/s of the same batch through the batched
+path and through the plain path (one call per system), p50/p99 latency of a
+batch build, the 10k-atom triclinic system at rc 6 against the CPU oracle,
+the roofline of the dominant kernel (pair fill) and the CPU baseline.
+
+Prints ONE JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neighborlist-rs_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+class DevArray:
+    def __init__(self, lib, host):
+        self.lib = lib
+        self.nbytes = host.nbytes
+        self.ptr = lib.nl_device_malloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise RuntimeError("device allocation failed")
+        if lib.nl_memcpy_h2d(C.c_void_p(self.ptr), host.ctypes.data_as(C.c_void_p),
+                             self.nbytes) != 0:
+            raise RuntimeError("H2D copy failed")
+
+    def free(self):
+        self.lib.nl_device_free(C.c_void_p(self.ptr))
+
+
+def run_systems(lib, L, dpos, n, sizes, H9, has, pbc, cut, multi):
+    r = C.c_void_p()
+    rc = lib.nl_search_batch_device(
+        C.c_void_p(dpos), n, sizes.ctypes.data_as(L.i64p), sizes.size,
+        H9.ctypes.data_as(L.dp), has.ctypes.data_as(L.u8p), pbc.ctypes.data_as(L.u8p),
+        cut.ctypes.data_as(L.dp), cut.size, 0, int(multi), 1, 0, 0, C.byref(r))
+    if rc != 0:
+        raise RuntimeError(L.last_error())
+    edges = sum(int(lib.nl_result_num_edges(r, q)) for q in range(cut.size))
+    lib.nl_result_free(r)
+    return edges
+
+
+def timing(lib):
+    b, c, f, t = C.c_float(), C.c_float(), C.c_float(), C.c_float()
+    e, nb = C.c_int64(), C.c_int64()
+    lib.nl_last_timing(C.byref(b), C.byref(c), C.byref(f), C.byref(t), C.byref(e), C.byref(nb))
+    return b.value, c.value, f.value, t.value, e.value
+
+
+def cell_arrays(cells):
+    S = len(cells)
+    H9 = np.zeros((S, 9))
+    has = np.zeros(S, np.uint8)
+    pbc = np.ones((S, 3), np.uint8)
+    for s, H in enumerate(cells):
+        if H is not None:
+            H9[s] = np.asarray(H).reshape(9)
+            has[s] = 1
+    return H9, has, pbc
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this engine has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from neighborlist_rs import _lib as L
+    from neighborlist_rs import workloads as W
+    lib = L.lib
+    lib.nl_set_timing(1)
+
+    # ---- headline: ragged batch, three cutoffs, one pass ------------------
+    pos, sizes, cells, cut = W.c4_ragged_batch()
+    cut = np.array(cut, dtype=np.float64)
+    H9, has, pbc = cell_arrays(cells)
+    n = pos.shape[0]
+    S = sizes.size
+    dpos = DevArray(lib, pos)
+    log(f"c4: {S} systems, {n} atoms, cutoffs {cut.tolist()}")
+    for _ in range(args.warmup):
+        run_systems(lib, L, dpos.ptr, n, sizes, H9, has, pbc, cut, True)
+    lat, fills, counts, builds = [], [], [], []
+    edges = 0
+    lib.nl_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        edges = run_systems(lib, L, dpos.ptr, n, sizes, H9, has, pbc, cut, True)
+        lat.append(time.perf_counter() - ts)
+        b, c, f, t, _ = timing(lib)
+        builds.append(b)
+        counts.append(c)
+        fills.append(f)
+    lib.nl_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = edges * args.steps / elapsed
+    sys_per_s = S * args.steps / elapsed
+    log(f"batched: {ms_per_step:.3f} ms/step, {edges} edges, build {np.mean(builds):.3f} "
+        f"count {np.mean(counts):.3f} fill {np.mean(fills):.3f} ms")
+
+    # roofline of the dominant kernel: the fill pass writes 28 B per edge
+    # (2 x int64 indices + 3 x int32 shift) and reads each position once
+    fill_ms = float(np.mean(fills))
+    alg_bytes = 28.0 * edges + 24.0 * n
+    achieved = alg_bytes / (fill_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_pairs<fill>", "kernel_ms": round(fill_ms, 4),
+                "alg_bytes_per_launch": int(alg_bytes)}
+
+    # ---- plain path: one call per system (same inputs, device-resident) --
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    one = np.ones(1, np.int64)
+    def plain_pass():
+        tot = 0
+        for s in range(S):
+            sz = np.array([sizes[s]], np.int64)
+            tot += run_systems(lib, L, dpos.ptr + 24 * int(starts[s]), int(sizes[s]), sz,
+                               np.ascontiguousarray(H9[s:s + 1]), has[s:s + 1].copy(),
+                               pbc[s:s + 1].copy(), cut, True)
+        return tot
+    plain_pass()
+    reps = max(1, min(5, args.steps // 4))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_edges = plain_pass()
+    plain_elapsed = time.perf_counter() - t0
+    plain_sys_per_s = S * reps / plain_elapsed
+    assert plain_edges == edges, (plain_edges, edges)
+    log(f"plain: {plain_sys_per_s:.1f} systems/s vs batched {sys_per_s:.1f}")
+
+    # ---- 10k-atom triclinic PBC at rc 6 (single system) ------------------
+    p3, H3, _ = W.c3_triclinic()
+    d3 = DevArray(lib, p3)
+    s3 = np.array([p3.shape[0]], np.int64)
+    c3 = np.array([6.0])
+    H3a = np.ascontiguousarray(H3.reshape(1, 9))
+    h1 = np.ones(1, np.uint8)
+    p1 = np.ones((1, 3), np.uint8)
+    for _ in range(3):
+        run_systems(lib, L, d3.ptr, p3.shape[0], s3, H3a, h1, p1, c3, False)
+    t0 = time.perf_counter()
+    k3 = 20
+    for _ in range(k3):
+        e3 = run_systems(lib, L, d3.ptr, p3.shape[0], s3, H3a, h1, p1, c3, False)
+    c3_ms = 1e3 * (time.perf_counter() - t0) / k3
+    log(f"c3 rc6: {c3_ms:.3f} ms, {e3} edges")
+
+    cpu = None
+    c3_cpu_ms = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = O.max_threads()
+        ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+        batch = np.repeat(np.arange(S, dtype=np.int32), sizes)
+        t0 = time.perf_counter()
+        passes = 0
+        cpu_edges = 0
+        while True:
+            res = O.search_batch_multi(pos, batch, ocells, cut.tolist(), False)
+            passes += 1
+            cpu_edges = sum(len(r[0]) for r in res)
+            if time.perf_counter() - t0 > args.cpu_seconds or passes >= 5:
+                break
+        cpu_elapsed = time.perf_counter() - t0
+        assert cpu_edges == edges, (cpu_edges, edges)
+        cpu = {"value": round(cpu_edges * passes / cpu_elapsed, 1), "unit": "edges/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{passes} full passes of the same 256-system batch, cutoffs "
+                         f"{{5,6,14}}, oracle/nl_oracle.c (OpenMP over systems)",
+               "systems_per_s": round(S * passes / cpu_elapsed, 1)}
+        t0 = time.perf_counter()
+        for _ in range(10):
+            O.search_single(p3, H3, [1, 1, 1], 6.0)
+        c3_cpu_ms = 1e3 * (time.perf_counter() - t0) / 10
+        log(f"cpu: {cpu['value']:.3e} edges/s on {threads} threads; c3 rc6 {c3_cpu_ms:.2f} ms")
+
+    out = {
+        "metric": "edges/s and systems/s (batched/served vs plain GPU); p50/p99 build latency",
+        "value": round(value, 1),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generators in neighborlist_rs/workloads.py)",
+        "config": {"workload": "c4_ragged_batch: 256 systems, N log-uniform [8,2048], half "
+                               "isolated / half periodic, cutoffs {5,6,14} in one pass",
+                   "atoms": int(n), "systems": int(S), "edges_per_step": int(edges)},
+        "systems_per_s": round(sys_per_s, 1),
+        "plain_gpu_systems_per_s": round(plain_sys_per_s, 1),
+        "batched_over_plain": round(sys_per_s / plain_sys_per_s, 2),
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 4),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 4)},
+        "stage_ms": {"build": round(float(np.mean(builds)), 4),
+                     "count": round(float(np.mean(counts)), 4),
+                     "fill": round(fill_ms, 4)},
+        "c3_triclinic_rc6": {"gpu_ms": round(c3_ms, 4), "edges": int(e3),
+                             "edges_per_s": round(e3 / (c3_ms * 1e-3), 1),
+                             "cpu_ms": None if c3_cpu_ms is None else round(c3_cpu_ms, 3),
+                             "speedup_vs_cpu": None if c3_cpu_ms is None
+                             else round(c3_cpu_ms / c3_ms, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    dpos.free()
+    d3.free()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+/*
+ * nl_engine_ext.h -- engine entry points of libnlmi355 that have no
+ * counterpart in neighborlist-rs.  They serve callers that keep data on the
+ * GPU (device-resident positions and outputs), the batched and served forms
+ * built on the plain path, and measurement.  Same error convention as
+ * nl_api.h (0 = success, message in nl_last_error()).
+ */
+#ifndef NL_ENGINE_EXT_H
+#define NL_ENGINE_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nl_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ragged batch whose positions are already in device memory.  sys_sizes
+ * (host) gives the atom count of each of n_sys consecutive systems.
+ * global_index != 0 offsets indices by the system start (batch semantics);
+ * 0 keeps each system's local indices (serving semantics).  want_ranges
+ * records per-system edge ranges (nl_result_system_ranges).  want_dist
+ * also emits the FP64 length |r_ij| of every edge (nl_result_copy_dist). */
+int nl_search_batch_device(const double* d_positions, size_t n, const int64_t* sys_sizes,
+                           size_t n_sys, const double* cells, const uint8_t* has_cell,
+                           const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs,
+                           int disjoint, int multi, int global_index, int want_ranges,
+                           int want_dist, nl_result** out);
+
+/* Host-input variant of the above (the batch entry point with explicit
+ * system sizes instead of batch ids, plus the range / distance options). */
+int nl_search_systems(const double* positions, size_t n, const int64_t* sys_sizes,
+                      size_t n_sys, const double* cells, const uint8_t* has_cell,
+                      const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs, int disjoint,
+                      int multi, int global_index, int want_ranges, int want_dist,
+                      nl_result** out);
+
+/* ranges[(s * n_lists + list) * 2 + {0,1}] = [first, end) edge of system s
+ * within list `list` (input cutoff order); mode[s] = 0 brute force on raw
+ * positions, 1 brute force minimum image, 2 cell list.  Either may be NULL. */
+int nl_result_system_ranges(const nl_result* r, int64_t* ranges, int32_t* modes);
+/* FP64 |r_ij| of list `list` into dist[M] (requires want_dist). */
+int nl_result_copy_dist(const nl_result* r, size_t list, double* dist);
+
+/* Stage timing with hipEvents on the engine stream (milliseconds). */
+void nl_set_timing(int on);
+int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* total_ms,
+                   int64_t* edges, int64_t* bins);
+
+/* Tunables: GPU bin edge = max cutoff / cell_div (1..4); FP32 filter with
+ * FP64 band recheck on the cell-list path (0/1).  Neither changes results. */
+void nl_set_cell_div(int d);
+int nl_get_cell_div(void);
+void nl_set_fp32_filter(int on);
+int nl_get_fp32_filter(void);
+
+/* Host-side cell helpers with the reference's arithmetic (no GPU needed):
+ * H^-1 by the cofactor formula of Cell::new (cell.rs:18-21; returns nonzero
+ * with "Cell matrix is not invertible" when det == 0) and Cell::wrap
+ * (cell.rs:52-72).  h is row-major H, lattice vectors as columns. */
+int nl_cell_inverse(const double* h, double* hinv);
+int nl_cell_wrap(const double* h, const uint8_t* pbc, const double* pos, double* out);
+
+/* Device memory plumbing for callers without a framework of their own
+ * (benchmarks, tests): plain hipMalloc / hipFree / synchronous copies. */
+void* nl_device_malloc(size_t bytes);
+int nl_device_free(void* p);
+int nl_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int nl_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int nl_device_synchronize(void);
+
+/* device selection for subsequent calls on this thread */
+int nl_set_device(int device);
+int nl_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NL_ENGINE_EXT_H */

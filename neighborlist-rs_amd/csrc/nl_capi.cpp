@@ -1,0 +1,425 @@
+// nl_capi.cpp -- extern "C" layer of libnlmi355 (include/nl_api.h,
+// include/nl_engine_ext.h).  Validation and segmentation follow the
+// reference's dispatch layer; all searching happens in the engine.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nl_api.h"
+#include "../../include/nl_engine_ext.h"
+#include "nl_engine.h"
+
+using nl::Engine;
+using nl::Request;
+using nl::Result;
+
+struct nl_result {
+  Result* r;
+};
+
+namespace {
+
+int wrap_result(Result* r, nl_result** out) {
+  *out = new nl_result{r};
+  return 0;
+}
+
+int run(Request& req, nl_result** out) {
+  if (!out) {
+    nl::set_error("output pointer is NULL");
+    return 1;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    nl::set_error("no HIP device available (libnlmi355 runs on the GPU only)");
+    return 1;
+  }
+  Engine* e = nl::engine_for_current_device();
+  Result* r = nullptr;
+  int rc;
+  {
+    std::lock_guard<std::mutex> g(e->mutex());
+    rc = e->run(req, &r);
+  }
+  if (rc != 0) return rc;
+  return wrap_result(r, out);
+}
+
+bool valid_cutoff(double c) { return std::isfinite(c) && c > 0.0; }
+
+// single.rs: one system, optional cell
+int single_request(const double* pos, size_t n, const double* h, const uint8_t* pbc,
+                   Request& req) {
+  req.pos = pos;
+  req.n = (int64_t)n;
+  req.starts = {0, (int64_t)n};
+  nl::SysIn s;
+  if (h) {
+    s.has_cell = 1;
+    std::memcpy(s.h, h, sizeof s.h);
+    for (int k = 0; k < 3; ++k) s.pbc[k] = pbc ? (pbc[k] != 0) : 1;
+  }
+  req.sys = {s};
+  return 0;
+}
+
+// batch.rs:25-46: systems are maximal runs of equal ids; cells checked.
+int batch_request(const double* pos, size_t n, const int32_t* batch, size_t n_cells,
+                  const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                  Request& req) {
+  req.pos = pos;
+  req.n = (int64_t)n;
+  req.starts.clear();
+  req.starts.push_back(0);
+  for (size_t i = 1; i < n; ++i)
+    if (batch[i] != batch[i - 1]) req.starts.push_back((int64_t)i);
+  req.starts.push_back((int64_t)n);
+  const size_t ns = req.starts.size() - 1;
+  if (has_cell && n_cells < ns) {
+    nl::set_error("Expected at least " + std::to_string(ns) + " cells, but got " +
+                  std::to_string(n_cells));
+    return 1;
+  }
+  req.sys.assign(ns, nl::SysIn());
+  if (has_cell) {
+    for (size_t s = 0; s < ns; ++s) {
+      if (!has_cell[s]) continue;
+      if (!cells) {
+        nl::set_error("cells is NULL while has_cell selects a cell");
+        return 1;
+      }
+      req.sys[s].has_cell = 1;
+      std::memcpy(req.sys[s].h, cells + 9 * s, sizeof(double) * 9);
+      for (int k = 0; k < 3; ++k) req.sys[s].pbc[k] = pbc ? (pbc[3 * s + k] != 0) : 1;
+    }
+  }
+  return 0;
+}
+
+int sizes_request(const double* pos, size_t n, const int64_t* sizes, size_t n_sys,
+                  const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                  Request& req) {
+  req.pos = pos;
+  req.n = (int64_t)n;
+  req.starts.assign(1, 0);
+  int64_t acc = 0;
+  for (size_t s = 0; s < n_sys; ++s) {
+    if (sizes[s] <= 0) {
+      nl::set_error("every system must hold at least one atom");
+      return 1;
+    }
+    acc += sizes[s];
+    req.starts.push_back(acc);
+  }
+  if (acc != (int64_t)n) {
+    nl::set_error("system sizes must add up to the number of positions");
+    return 1;
+  }
+  req.sys.assign(n_sys, nl::SysIn());
+  if (has_cell) {
+    for (size_t s = 0; s < n_sys; ++s) {
+      if (!has_cell[s]) continue;
+      req.sys[s].has_cell = 1;
+      std::memcpy(req.sys[s].h, cells + 9 * s, sizeof(double) * 9);
+      for (int k = 0; k < 3; ++k) req.sys[s].pbc[k] = pbc ? (pbc[3 * s + k] != 0) : 1;
+    }
+  }
+  return 0;
+}
+
+int empty_result(int k, nl_result** out) {
+  Result* r = new Result();
+  r->eng = nullptr;
+  r->k = k;
+  r->len.assign(k, 0);
+  r->base.assign(k, 0);
+  return wrap_result(r, out);
+}
+
+int check_cutoffs(const double* c, size_t k, const char* msg) {
+  for (size_t q = 0; q < k; ++q)
+    if (!valid_cutoff(c[q])) {
+      nl::set_error(msg);
+      return 1;
+    }
+  if (k > (size_t)nl::kMaxCutoffs) {
+    nl::set_error("at most 64 cutoffs are supported");
+    return 1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nl_search_single(const double* positions, size_t n, const double* h, const uint8_t* pbc,
+                     double cutoff, int parallel, nl_result** out) {
+  (void)parallel;
+  if (check_cutoffs(&cutoff, 1, "Cutoff must be positive and finite")) return 1;
+  if (n == 0) return empty_result(1, out);  // single.rs:14-17
+  Request req;
+  single_request(positions, n, h, pbc, req);
+  req.cutoffs = {cutoff};
+  return run(req, out);
+}
+
+int nl_search_single_multi(const double* positions, size_t n, const double* h,
+                           const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs,
+                           int disjoint, nl_result** out) {
+  if (check_cutoffs(cutoffs, n_cutoffs, "All cutoffs must be positive and finite")) return 1;
+  if (n == 0) return empty_result((int)n_cutoffs, out);  // single.rs:99-101
+  if (n_cutoffs == 0) return empty_result(0, out);       // single.rs:102-104
+  Request req;
+  single_request(positions, n, h, pbc, req);
+  req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
+  req.multi = true;
+  req.disjoint = disjoint != 0;
+  return run(req, out);
+}
+
+int nl_search_batch(const double* positions, size_t n, const int32_t* batch, size_t n_cells,
+                    const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                    double cutoff, int parallel, nl_result** out) {
+  (void)parallel;
+  if (check_cutoffs(&cutoff, 1, "Cutoff must be positive and finite")) return 1;
+  if (n == 0) return empty_result(1, out);  // batch.rs:21-23
+  Request req;
+  if (batch_request(positions, n, batch, n_cells, cells, has_cell, pbc, req)) return 1;
+  req.cutoffs = {cutoff};
+  return run(req, out);
+}
+
+int nl_search_batch_multi(const double* positions, size_t n, const int32_t* batch,
+                          size_t n_cells, const double* cells, const uint8_t* has_cell,
+                          const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs,
+                          int disjoint, nl_result** out) {
+  if (check_cutoffs(cutoffs, n_cutoffs, "All cutoffs must be positive and finite")) return 1;
+  if (n == 0 || n_cutoffs == 0) return empty_result((int)n_cutoffs, out);  // batch.rs:160-166
+  Request req;
+  if (batch_request(positions, n, batch, n_cells, cells, has_cell, pbc, req)) return 1;
+  req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
+  req.multi = true;
+  req.disjoint = disjoint != 0;
+  return run(req, out);
+}
+
+static int systems_common(bool device, const double* positions, size_t n,
+                          const int64_t* sys_sizes, size_t n_sys, const double* cells,
+                          const uint8_t* has_cell, const uint8_t* pbc, const double* cutoffs,
+                          size_t n_cutoffs, int disjoint, int multi, int global_index,
+                          int want_ranges, int want_dist, nl_result** out) {
+  if (check_cutoffs(cutoffs, n_cutoffs, "All cutoffs must be positive and finite")) return 1;
+  if (n == 0 || n_cutoffs == 0) return empty_result((int)n_cutoffs, out);
+  Request req;
+  if (sizes_request(positions, n, sys_sizes, n_sys, cells, has_cell, pbc, req)) return 1;
+  req.pos_device = device;
+  req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
+  req.multi = multi != 0;
+  req.disjoint = disjoint != 0;
+  req.global_index = global_index != 0;
+  req.want_ranges = want_ranges != 0;
+  req.want_dist = want_dist != 0;
+  return run(req, out);
+}
+
+int nl_search_batch_device(const double* d_positions, size_t n, const int64_t* sys_sizes,
+                           size_t n_sys, const double* cells, const uint8_t* has_cell,
+                           const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs,
+                           int disjoint, int multi, int global_index, int want_ranges,
+                           int want_dist, nl_result** out) {
+  return systems_common(true, d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs,
+                        n_cutoffs, disjoint, multi, global_index, want_ranges, want_dist, out);
+}
+
+int nl_search_systems(const double* positions, size_t n, const int64_t* sys_sizes,
+                      size_t n_sys, const double* cells, const uint8_t* has_cell,
+                      const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs, int disjoint,
+                      int multi, int global_index, int want_ranges, int want_dist,
+                      nl_result** out) {
+  return systems_common(false, positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs,
+                        n_cutoffs, disjoint, multi, global_index, want_ranges, want_dist, out);
+}
+
+size_t nl_result_num_lists(const nl_result* r) { return r ? (size_t)r->r->k : 0; }
+
+int64_t nl_result_num_edges(const nl_result* r, size_t list) {
+  if (!r || list >= (size_t)r->r->k) return -1;
+  return r->r->len[list];
+}
+
+int nl_result_copy(const nl_result* r, size_t list, int64_t* edge_i, int64_t* edge_j,
+                   int32_t* shifts) {
+  if (!r || list >= (size_t)r->r->k) {
+    nl::set_error("invalid result or list index");
+    return 1;
+  }
+  const Result* R = r->r;
+  const int64_t m = R->len[list];
+  if (m == 0) return 0;
+  const int64_t b = R->base[list];
+  hipStream_t st = R->eng->stream();
+  const int64_t* e = (const int64_t*)R->edges;
+  hipError_t err = hipSuccess;
+  if (edge_i && err == hipSuccess)
+    err = hipMemcpyAsync(edge_i, e + 2 * b, 8 * m, hipMemcpyDeviceToHost, st);
+  if (edge_j && err == hipSuccess)
+    err = hipMemcpyAsync(edge_j, e + 2 * b + m, 8 * m, hipMemcpyDeviceToHost, st);
+  if (shifts && err == hipSuccess)
+    err = hipMemcpyAsync(shifts, (const int32_t*)R->shifts + 3 * b, 12 * m,
+                         hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (err != hipSuccess) {
+    nl::set_error(std::string("HIP error in nl_result_copy: ") + hipGetErrorString(err));
+    return 1;
+  }
+  return 0;
+}
+
+int nl_result_copy_dist(const nl_result* r, size_t list, double* dist) {
+  if (!r || list >= (size_t)r->r->k || !r->r->dist) {
+    nl::set_error("no distances recorded for this result");
+    return 1;
+  }
+  const Result* R = r->r;
+  const int64_t m = R->len[list];
+  if (m == 0) return 0;
+  hipStream_t st = R->eng->stream();
+  hipError_t err = hipMemcpyAsync(dist, (const double*)R->dist + R->base[list], 8 * m,
+                                  hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (err != hipSuccess) {
+    nl::set_error(std::string("HIP error in nl_result_copy_dist: ") + hipGetErrorString(err));
+    return 1;
+  }
+  return 0;
+}
+
+int nl_result_device(const nl_result* r, size_t list, const int64_t** edge_index,
+                     const int32_t** shifts) {
+  if (!r || list >= (size_t)r->r->k) {
+    nl::set_error("invalid result or list index");
+    return 1;
+  }
+  const Result* R = r->r;
+  const int64_t b = R->base[list];
+  if (edge_index) *edge_index = R->edges ? (const int64_t*)R->edges + 2 * b : nullptr;
+  if (shifts) *shifts = R->shifts ? (const int32_t*)R->shifts + 3 * b : nullptr;
+  return 0;
+}
+
+int nl_result_system_ranges(const nl_result* r, int64_t* ranges, int32_t* modes) {
+  if (!r) {
+    nl::set_error("invalid result");
+    return 1;
+  }
+  const Result* R = r->r;
+  if (R->ranges.empty() && R->total > 0) {
+    nl::set_error("system ranges were not requested for this result");
+    return 1;
+  }
+  if (ranges && !R->ranges.empty())
+    std::memcpy(ranges, R->ranges.data(), 8 * R->ranges.size());
+  if (modes && !R->modes.empty()) std::memcpy(modes, R->modes.data(), 4 * R->modes.size());
+  return 0;
+}
+
+void nl_result_free(nl_result* r) {
+  if (!r) return;
+  delete r->r;
+  delete r;
+}
+
+size_t nl_get_brute_force_threshold(void) { return nl::config().bf_threshold.load(); }
+void nl_set_brute_force_threshold(size_t v) { nl::config().bf_threshold.store(v); }
+size_t nl_get_parallel_threshold(void) { return nl::config().par_threshold.load(); }
+void nl_set_parallel_threshold(size_t v) { nl::config().par_threshold.store(v); }
+size_t nl_get_stack_threshold(void) { return nl::config().stack_threshold.load(); }
+void nl_set_stack_threshold(size_t v) {
+  nl::config().stack_threshold.store(v < 1024 ? v : 1024);  // MAX_STACK_SIZE (config.rs:4)
+}
+
+const char* nl_last_error(void) { return nl::last_error(); }
+const char* nl_version(void) { return "nlmi355 0.1.0 (gfx950)"; }
+
+void nl_set_timing(int on) { nl::engine_for_current_device()->set_timing(on != 0); }
+
+int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* total_ms,
+                   int64_t* edges, int64_t* bins) {
+  nl::StageTimes t = nl::engine_for_current_device()->last_times();
+  if (build_ms) *build_ms = t.build;
+  if (count_ms) *count_ms = t.count;
+  if (fill_ms) *fill_ms = t.fill;
+  if (total_ms) *total_ms = t.total;
+  if (edges) *edges = t.edges;
+  if (bins) *bins = t.bins;
+  return 0;
+}
+
+void nl_set_cell_div(int d) { nl::config().cell_div.store(d < 1 ? 1 : (d > 4 ? 4 : d)); }
+int nl_get_cell_div(void) { return nl::config().cell_div.load(); }
+void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }
+int nl_get_fp32_filter(void) { return nl::config().use_fp32.load(); }
+
+int nl_cell_inverse(const double* h, double* hinv) {
+  if (!nl::inverse3(h, hinv)) {
+    nl::set_error("Cell matrix is not invertible");  // cell.rs:6
+    return 1;
+  }
+  return 0;
+}
+
+int nl_cell_wrap(const double* h, const uint8_t* pbc, const double* pos, double* out) {
+  double inv[9];
+  if (nl_cell_inverse(h, inv)) return 1;
+  double f[3];
+  nl::matvec3(inv, pos[0], pos[1], pos[2], f);
+  for (int k = 0; k < 3; ++k)
+    if (!pbc || pbc[k]) f[k] = f[k] - std::floor(f[k]);
+  nl::matvec3(h, f[0], f[1], f[2], out);
+  return 0;
+}
+
+void* nl_device_malloc(size_t bytes) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    nl::set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  return p;
+}
+
+static int hip_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  nl::set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return 1;
+}
+
+int nl_device_free(void* p) { return hip_status(hipFree(p), "hipFree"); }
+int nl_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+int nl_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+int nl_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int nl_set_device(int device) {
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    nl::set_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return 1;
+  }
+  return 0;
+}
+
+int nl_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// nl_kernels.h -- launch wrappers for the gfx950 kernels in nl_kernels.hip.
+#pragma once
+#include "nl_common.h"
+
+namespace nl {
+
+struct alignas(16) Pos64 {  // sorted wrapped position + local original index
+  double x, y, z, w;
+};
+
+// Device pointers handed to the pair kernels.
+struct PairArgs {
+  const double* pos;         // raw positions (N,3), f64
+  const SysDesc* sys;
+  const int64_t* sys_start;  // n_sys + 1
+  const uint32_t* keys;      // sorted bin keys
+  const int32_t* cell_start; // total_bins + 1
+  const Pos64* spos64;       // sorted wrapped positions (FP64), w = local index
+  const float4* spos32;      // sorted wrapped positions (FP32), w = local index bits
+  const int4* sidx;          // sorted wrap shifts s_a, w = local index
+  const double* rc2;         // k squared cutoffs, ascending
+  const float2* band;        // [n_sys][k] FP32 accept/reject thresholds
+  uint32_t* counts;          // [k][n] (count pass)
+  const int64_t* offsets;    // [k][n] + 1 (fill pass)
+  const int64_t* bounds;     // [k + 1] first edge of each list (sorted-cutoff order)
+  int64_t* edges;            // list q: sources at base_q + e, targets at base_q + M_q + e
+  int32_t* shifts;           // list q: 3 per edge at 3 e
+  double* dist;              // optional |r_ij| per edge (FP64), list q at e
+  int64_t n;
+  int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index, pad;
+};
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
+                       const Params& p, hipStream_t st);
+hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
+                      int64_t n, uint32_t* keys, uint32_t* vals, uint32_t* hist, int passes,
+                      hipStream_t st);
+hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                             uint32_t* vout, int64_t n, int shift, const uint32_t* hist,
+                             uint32_t* status, uint32_t* ticket, hipStream_t st);
+hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const double* pos,
+                           const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,
+                           int64_t total_bins, int32_t* cell_start, Pos64* spos64,
+                           float4* spos32, int4* sidx, hipStream_t st);
+hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st);
+hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
+                       int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st);
+hipError_t launch_sys_ranges(const int64_t* offsets, const SysDesc* sys, int n_sys, int k,
+                             int64_t n, int64_t* ranges, hipStream_t st);
+
+inline int64_t radix_tiles(int64_t n) { return (n + kRadixTile - 1) / kRadixTile; }
+
+}  // namespace nl

@@ -1,0 +1,837 @@
+// nl_kernels.hip -- gfx950 kernels of the neighbour-list engine.
+//
+// Pipeline per call (one launch per stage for the whole ragged batch):
+//   plan      one block per system: auto-box span, H^-1, widths, dispatch
+//             (single.rs:19-89), GPU bin grid, FP32 band thresholds
+//   bin       one thread per atom: wrapped fractional coordinate -> bin key,
+//             global digit histograms for the radix passes
+//   radix     LSD radix sort of (key, atom) pairs, 8-bit digits, one launch
+//             per digit with decoupled look-back between tiles (stable)
+//   finalize  one thread per sorted slot: cell offsets from key boundaries,
+//             sorted wrapped positions (FP64 + FP32) and wrap shifts
+//   pairs     one wavefront per unit (a brute-force row or a sorted atom):
+//             stencil segments staged in LDS, 64 candidates per step,
+//             ballot-compacted output; count pass then fill pass
+//   scan      exclusive scan of per-unit counts (decoupled look-back)
+//
+// Compiled with -ffp-contract=off: every FP64 expression that feeds a
+// decision is the reference's, term for term.
+#include <hip/hip_runtime.h>
+
+#include "nl_kernels.h"
+
+namespace nl {
+
+namespace {
+
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPre = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1;
+constexpr uint64_t kFlagAgg64 = 1ull << 62;
+constexpr uint64_t kFlagPre64 = 2ull << 62;
+constexpr uint64_t kValMask64 = (1ull << 62) - 1;
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ inline int find_sys(const int64_t* sys_start, int n_sys, int64_t a) {
+  int lo = 0, hi = n_sys - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (sys_start[mid] <= a)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__device__ inline T wave_incl_scan(T v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ inline T atomic_load_agent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline void atomic_store_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan over 256 values held one per thread (block of 256).
+__device__ inline uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*4*/,
+                                               uint32_t* total) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) s_tmp[w] = incl;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int i = 0; i < w; ++i) pre += s_tmp[i];
+  if (total) *total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+  __syncthreads();
+  return pre + incl - v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Planning: one block per system.  Restates single.rs:19-64 (auto-box,
+// Cell::new, perpendicular widths, mic_safe, brute-force dispatch) and picks
+// the GPU bin grid.  The bin grid never changes which pairs are found: the
+// cell-list distance uses wrapped positions and the image shift only
+// (search.rs:540-549, 695-697), so any grid with a covering stencil gives
+// the reference's edge set.
+__global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
+                                              SysDesc* __restrict__ sys,
+                                              float2* __restrict__ band,
+                                              const double* __restrict__ rc2, Params p) {
+  const int s = blockIdx.x;
+  SysDesc* d = sys + s;
+  const int64_t a0 = d->atom_start;
+  const int n = d->n_atoms;
+  __shared__ double s_mn[3][4], s_mx[3][4];
+  if (!d->has_cell) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const double* q = pos + 3 * (a0 + i);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double v = q[k];
+        mn[k] = v < mn[k] ? v : mn[k];
+        mx[k] = v > mx[k] ? v : mx[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      for (int off = 32; off > 0; off >>= 1) {
+        double a = __shfl_xor(mn[k], off, 64), b = __shfl_xor(mx[k], off, 64);
+        mn[k] = a < mn[k] ? a : mn[k];
+        mx[k] = b > mx[k] ? b : mx[k];
+      }
+    }
+    if ((threadIdx.x & 63) == 0) {
+      for (int k = 0; k < 3; ++k) {
+        s_mn[k][threadIdx.x >> 6] = mn[k];
+        s_mx[k][threadIdx.x >> 6] = mx[k];
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double h[9];
+  if (!d->has_cell) {
+    const double margin = p.rc_max + 1.0;  // AUTO_BOX_MARGIN (single.rs:6)
+    for (int k = 0; k < 9; ++k) h[k] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double mn = s_mn[k][0], mx = s_mx[k][0];
+      for (int w = 1; w < 4; ++w) {
+        mn = s_mn[k][w] < mn ? s_mn[k][w] : mn;
+        mx = s_mx[k][w] > mx ? s_mx[k][w] : mx;
+      }
+      h[4 * k] = (mx - mn) + 2.0 * margin;
+    }
+    for (int k = 0; k < 3; ++k) d->pbc[k] = 0;
+  } else {
+    for (int k = 0; k < 9; ++k) h[k] = d->h[k];
+  }
+  double hinv[9];
+  if (!inverse3(h, hinv)) {
+    d->status = 1;
+    d->mode = kBruteIsolated;
+    for (int k = 0; k < 3; ++k) {
+      d->nb[k] = 1;
+      d->ns[k] = 0;
+    }
+    return;
+  }
+  d->status = 0;
+  double w[3];
+  widths3(hinv, w);
+  for (int k = 0; k < 9; ++k) {
+    d->h[k] = h[k];
+    d->hinv[k] = hinv[k];
+    d->h32[k] = (float)h[k];
+  }
+  for (int k = 0; k < 3; ++k) d->w[k] = w[k];
+  const double minw = fmin(fmin(w[0], w[1]), w[2]);
+  const bool mic_safe = p.rc_max * 2.0 < minw;
+  const uint64_t thr = ((uint64_t)(uint32_t)p.bf_threshold_hi << 32) | (uint32_t)p.bf_threshold_lo;
+  const bool any_pbc = d->pbc[0] || d->pbc[1] || d->pbc[2];
+  int mode;
+  if ((uint64_t)n < thr && mic_safe)
+    mode = (p.multi || any_pbc) ? kBruteMic : kBruteIsolated;
+  else
+    mode = kCellList;
+  d->mode = mode;
+  if (mode != kCellList) {
+    for (int k = 0; k < 3; ++k) {
+      d->nb[k] = 1;
+      d->ns[k] = 0;
+    }
+    d->delta = 0.0;
+    return;
+  }
+  // GPU bin grid: edge ~ rc_max / cell_div, at most bin_cap bins.
+  const double edge = p.rc_max / (double)p.cell_div;
+  int nb[3];
+  for (int k = 0; k < 3; ++k) {
+    double v = floor(w[k] / edge);
+    nb[k] = v < 1.0 ? 1 : (v > 4096.0 ? 4096 : (int)v);
+  }
+  const int64_t cap = d->bin_cap;
+  int64_t prod = (int64_t)nb[0] * nb[1] * nb[2];
+  if (prod > cap) {
+    double f = cbrt((double)cap / (double)prod);
+    for (int k = 0; k < 3; ++k) {
+      int v = (int)(nb[k] * f);
+      nb[k] = v < 1 ? 1 : v;
+    }
+    prod = (int64_t)nb[0] * nb[1] * nb[2];
+    while (prod > cap) {
+      int kk = 0;
+      for (int k = 1; k < 3; ++k)
+        if (nb[k] > nb[kk]) kk = k;
+      nb[kk] -= 1;
+      prod = (int64_t)nb[0] * nb[1] * nb[2];
+    }
+  }
+  int ns[3];
+  for (int k = 0; k < 3; ++k) {
+    // covering reach: ceil(rc n / w), widened by one bin when the ratio sits
+    // within 1e-9 of an integer (floating-point slack at the boundary)
+    double r = p.rc_max * (double)nb[k] / w[k];
+    ns[k] = (int)ceil(r + 1e-9);
+    d->nb[k] = nb[k];
+    d->ns[k] = ns[k];
+  }
+  // FP32 filter band: bound on |d32 - d64| for any candidate (see the
+  // pair kernel).  P bounds |wrapped coordinate|, O bounds |image offset|.
+  double P = 0.0, O = 0.0;
+  for (int r = 0; r < 3; ++r) {
+    double pr = 0.0, orr = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double a = fabs(h[3 * r + c]);
+      pr += a;
+      orr += a * ((double)ns[c] / (double)nb[c] + 2.0);
+    }
+    P = fmax(P, pr);
+    O = fmax(O, orr);
+  }
+  const double delta = 64.0 * 5.9604644775390625e-08 * (P + O + p.rc_max);
+  d->delta = delta;
+  for (int q = 0; q < p.k; ++q) {
+    double rc = sqrt(rc2[q]);
+    double lo = rc - delta;
+    lo = lo > 0.0 ? lo * lo : 0.0;
+    double hi = (rc + delta) * (rc + delta);
+    // round the thresholds outward so the certain-accept / certain-reject
+    // decisions stay conservative
+    band[(int64_t)s * p.k + q] = make_float2(__double2float_rd(lo), __double2float_ru(hi));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binning: one thread per atom.  Key = bin_base + linear bin (cell-list
+// systems) or bin_base (brute-force systems keep their input order).
+__global__ __launch_bounds__(256) void k_bin(const double* __restrict__ pos,
+                                             const SysDesc* __restrict__ sys,
+                                             const int64_t* __restrict__ sys_start, int n_sys,
+                                             int64_t n, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ vals,
+                                             uint32_t* __restrict__ hist, int passes) {
+  __shared__ uint32_t lh[4][256];
+  for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) (&lh[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) {
+    const int s = find_sys(sys_start, n_sys, a);
+    const SysDesc* d = sys + s;
+    uint32_t key = (uint32_t)d->bin_base;
+    if (d->mode == kCellList) {
+      const double* q = pos + 3 * a;
+      double f[3];
+      matvec3(d->hinv, q[0], q[1], q[2], f);
+      int b[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double u = f[k] - floor(f[k]);
+        int v = (int)(u * (double)d->nb[k]);  // u >= 0
+        b[k] = v < d->nb[k] - 1 ? v : d->nb[k] - 1;
+      }
+      key += (uint32_t)(b[0] + d->nb[0] * (b[1] + d->nb[1] * b[2]));
+    }
+    keys[a] = key;
+    vals[a] = (uint32_t)a;
+    for (int ps = 0; ps < passes; ++ps) atomicAdd(&lh[ps][(key >> (kRadixBits * ps)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int ps = 0; ps < passes; ++ps) {
+    uint32_t c = lh[ps][threadIdx.x];
+    if (c) atomicAdd(&hist[ps * 256 + threadIdx.x], c);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One stable LSD radix pass over 8 bits.  Tile = 2048 keys; tiles are taken
+// in ticket order, so every tile a block waits on has already published its
+// per-digit count and the look-back always terminates.
+__global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__ kin,
+                                                    const uint32_t* __restrict__ vin,
+                                                    uint32_t* __restrict__ kout,
+                                                    uint32_t* __restrict__ vout, int64_t n,
+                                                    int shift, const uint32_t* __restrict__ hist,
+                                                    uint32_t* __restrict__ status,
+                                                    uint32_t* __restrict__ ticket) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_cnt[4][256];
+  __shared__ uint32_t s_excl[256];
+  __shared__ uint32_t s_tmp[4];
+  const int tid = threadIdx.x, w = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  const uint32_t hv = hist[tid];
+  const uint32_t base_d = block_excl_scan_256(hv, s_tmp, nullptr);
+  s_base[tid] = base_d;
+  s_run[tid] = 0;
+  for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int64_t t0 = (int64_t)tile * kRadixTile;
+  uint32_t kk[8], vv[8], lp[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int64_t e = t0 + r * 256 + tid;
+    const bool valid = e < n;
+    const uint32_t key = valid ? kin[e] : 0u;
+    const uint32_t dig = (key >> shift) & 255u;
+    kk[r] = key;
+    vv[r] = valid ? vin[e] : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (dig >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = lanes_below(peers);
+    if (valid && rank == 0) s_cnt[w][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = s_run[dig];
+      for (int i = 0; i < w; ++i) pre += s_cnt[i][dig];
+      lp[r] = pre + rank;
+    }
+    __syncthreads();
+    s_run[tid] += s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
+    s_cnt[0][tid] = s_cnt[1][tid] = s_cnt[2][tid] = s_cnt[3][tid] = 0;
+    __syncthreads();
+  }
+  // decoupled look-back, one digit per thread
+  {
+    const uint32_t cnt = s_run[tid];
+    uint32_t* st = status + (int64_t)tile * 256 + tid;
+    uint32_t excl = 0;
+    if (tile == 0) {
+      atomic_store_agent(st, kFlagPre | cnt);
+    } else {
+      atomic_store_agent(st, kFlagAgg | cnt);
+      int64_t t = (int64_t)tile - 1;
+      while (true) {
+        const uint32_t v = atomic_load_agent(status + t * 256 + tid);
+        const uint32_t flag = v & ~kValMask;
+        if (flag == 0) continue;  // predecessor still ranking its tile
+        excl += v & kValMask;
+        if (flag == kFlagPre) break;
+        --t;
+      }
+      atomic_store_agent(st, kFlagPre | (excl + cnt));
+    }
+    s_excl[tid] = excl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int64_t e = t0 + r * 256 + tid;
+    if (e < n) {
+      const uint32_t dig = (kk[r] >> shift) & 255u;
+      const uint32_t dst = s_base[dig] + s_excl[dig] + lp[r];
+      kout[dst] = kk[r];
+      vout[dst] = vv[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: one thread per sorted slot p.  Cell offsets from key boundaries;
+// wrapped position and wrap shift of the atom, as CellList::build computes
+// them (search.rs:189-204): s = -floor(H^-1 r), wrapped = r - H (-s).
+__global__ __launch_bounds__(256) void k_finalize(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const double* __restrict__ pos, const SysDesc* __restrict__ sys,
+    const int64_t* __restrict__ sys_start, int n_sys, int64_t n, int64_t total_bins,
+    int32_t* __restrict__ cell_start, Pos64* __restrict__ spos64, float4* __restrict__ spos32,
+    int4* __restrict__ sidx) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t key = keys[p];
+  const int64_t prev = p > 0 ? (int64_t)keys[p - 1] : -1;
+  for (int64_t b = prev + 1; b <= key; ++b) cell_start[b] = (int32_t)p;
+  if (p == n - 1)
+    for (int64_t b = key + 1; b <= total_bins; ++b) cell_start[b] = (int32_t)n;
+  const int64_t a = vals[p];
+  const int s = find_sys(sys_start, n_sys, a);
+  const SysDesc* d = sys + s;
+  if (d->mode != kCellList) return;
+  const double* q = pos + 3 * a;
+  double f[3];
+  matvec3(d->hinv, q[0], q[1], q[2], f);
+  int32_t si[3];
+  for (int k = 0; k < 3; ++k) si[k] = as_i32(-floor(f[k]));
+  double sv[3];
+  matvec3(d->h, (double)(-si[0]), (double)(-si[1]), (double)(-si[2]), sv);
+  const double wx = q[0] - sv[0], wy = q[1] - sv[1], wz = q[2] - sv[2];
+  const int32_t loc = (int32_t)(a - d->atom_start);
+  spos64[p] = Pos64{wx, wy, wz, (double)loc};
+  spos32[p] = make_float4((float)wx, (float)wy, (float)wz, __int_as_float(loc));
+  sidx[p] = make_int4(si[0], si[1], si[2], loc);
+}
+
+// ---------------------------------------------------------------------------
+// Pair search.  One wavefront per unit; a block holds kWaves units.
+namespace {
+
+struct WaveLds {
+  int32_t incl[64];   // inclusive candidate prefix per stencil segment
+  int32_t start[64];  // first sorted slot of the segment's bin
+  int32_t bs[3][64];  // bin-image shift of the segment
+  double off[3][64];  // H bs in FP64, reference order
+  float off32[3][64]; // H bs in FP32 (filter)
+};
+
+// index of the first list a squared distance falls in (cutoffs sorted
+// ascending): the number of cutoffs it is not strictly below
+__device__ inline int first_list(double d2, const double* __restrict__ rc2, int k) {
+  int q = 0;
+  for (int i = 0; i < k; ++i) q += !(d2 < rc2[i]);
+  return q;
+}
+
+// Emit or count one step of up to 64 candidates.  hit = valid & pair rule,
+// qs = first list reached.  `cur` holds the wave's running counts (count
+// pass) or write cursors (fill pass) for lists q_lo .. q_lo+q_cnt-1.
+template <bool FILL>
+__device__ inline void emit_step(const PairArgs& A, bool hit, int qs, int64_t gi, int64_t gj,
+                                 int sx, int sy, int sz, double dd, int64_t* cur) {
+#pragma unroll
+  for (int g = 0; g < kGroup; ++g) {
+    if (g >= A.q_cnt) break;
+    const int q = A.q_lo + g;
+    const bool mem = hit && (A.disjoint ? (qs == q) : (qs <= q));
+    const uint64_t m = __ballot(mem);
+    if (m == 0) continue;
+    if (FILL) {
+      if (mem) {
+        const int64_t e = cur[g] + lanes_below(m);
+        const int64_t base = A.bounds[q], len = A.bounds[q + 1] - base;
+        A.edges[base + e] = gi;
+        A.edges[base + len + e] = gj;
+        A.shifts[3 * e] = sx;
+        A.shifts[3 * e + 1] = sy;
+        A.shifts[3 * e + 2] = sz;
+        if (A.dist) A.dist[e] = dd;
+      }
+    }
+    cur[g] += (int64_t)__popcll(m);
+  }
+}
+
+template <bool FILL>
+__device__ void unit_bruteforce(const PairArgs& A, const SysDesc* d, int64_t u, int mode,
+                                int64_t* cur) {
+  const int lane = lane_id();
+  const int64_t a0 = d->atom_start;
+  const int n = d->n_atoms;
+  const int i = (int)(u - a0);
+  const double* pi = A.pos + 3 * u;
+  const double xi = pi[0], yi = pi[1], zi = pi[2];
+  const int64_t gi = A.global_index ? u : (int64_t)i;
+  const int pb0 = d->pbc[0], pb1 = d->pbc[1], pb2 = d->pbc[2];
+  for (int j0 = i + 1; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    const bool valid = j < n;
+    double d2 = INFINITY, dx = 0, dy = 0, dz = 0;
+    int sx = 0, sy = 0, sz = 0;
+    if (valid) {
+      const double* pj = A.pos + 3 * (a0 + j);
+      const double xj = pj[0], yj = pj[1], zj = pj[2];
+      if (mode == kBruteIsolated) {
+        // search.rs:1254-1258, 1278-1281
+        dx = xj - xi;
+        dy = yj - yi;
+        dz = zj - zi;
+      } else {
+        // Cell::get_shift_and_displacement (cell.rs:74-94)
+        double df[3];
+        matvec3(d->hinv, xj - xi, yj - yi, zj - zi, df);
+        const double f0 = pb0 ? -round(df[0]) : 0.0;
+        const double f1 = pb1 ? -round(df[1]) : 0.0;
+        const double f2 = pb2 ? -round(df[2]) : 0.0;
+        sx = as_i32(f0);
+        sy = as_i32(f1);
+        sz = as_i32(f2);
+        double v[3];
+        matvec3(d->h, f0, f1, f2, v);
+        dx = (xj + v[0]) - xi;
+        dy = (yj + v[1]) - yi;
+        dz = (zj + v[2]) - zi;
+      }
+      d2 = norm2(dx, dy, dz);
+    }
+    const int qs = first_list(d2, A.rc2, A.k);
+    const bool hit = valid && qs < A.k;
+    if (__ballot(hit) == 0) continue;
+    const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
+    emit_step<FILL>(A, hit, qs, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
+  }
+}
+
+template <bool FILL>
+__device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_t p,
+                              WaveLds& L, int64_t* cur) {
+  const int lane = lane_id();
+  const int64_t a0 = d->atom_start;
+  const int nbx = d->nb[0], nby = d->nb[1], nbz = d->nb[2];
+  const int nsx = d->ns[0], nsy = d->ns[1], nsz = d->ns[2];
+  const int64_t bb = d->bin_base;
+  const int lb = (int)((int64_t)A.keys[p] - bb);
+  const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
+  const Pos64 pi = A.spos64[p];
+  const int4 si = A.sidx[p];
+  const int oi = si.w;
+  const float4 pi32 = A.spos32[p];
+  const int64_t gi = A.global_index ? a0 + oi : (int64_t)oi;
+  const double h0 = d->h[0], h1 = d->h[1], h2 = d->h[2], h3 = d->h[3], h4 = d->h[4],
+               h5 = d->h[5], h6 = d->h[6], h7 = d->h[7], h8 = d->h[8];
+  const float* h32 = d->h32;
+  const int wx = 2 * nsx + 1, wy = 2 * nsy + 1, wz = 2 * nsz + 1;
+  const int T = wx * wy * wz;
+  const float2* band = A.band + (int64_t)s * A.k;
+  const float rej32 = band[A.k - 1].y;  // beyond the largest band: certain miss
+  for (int t0 = 0; t0 < T; t0 += 64) {
+    const int t = t0 + lane;
+    int len = 0, st = 0, bsx = 0, bsy = 0, bsz = 0;
+    double ox = 0, oy = 0, oz = 0;
+    if (t < T) {
+      const int dz = t % wz - nsz;
+      const int rest = t / wz;
+      const int dy = rest % wy - nsy;
+      const int dx = rest / wy - nsx;
+      int cx, cy, cz;
+      div_mod(bx + dx, nbx, &cx, &bsx);
+      div_mod(by + dy, nby, &cy, &bsy);
+      div_mod(bz + dz, nbz, &cz, &bsz);
+      const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
+      st = A.cell_start[bin];
+      len = A.cell_start[bin + 1] - st;
+      // (ox + oy) + oz of search.rs:498 = H (bsx, bsy, bsz) in gemv order
+      ox = (h0 * (double)bsx + h1 * (double)bsy) + h2 * (double)bsz;
+      oy = (h3 * (double)bsx + h4 * (double)bsy) + h5 * (double)bsz;
+      oz = (h6 * (double)bsx + h7 * (double)bsy) + h8 * (double)bsz;
+    }
+    const int incl = wave_incl_scan(len);
+    L.incl[lane] = incl;
+    L.start[lane] = st;
+    L.bs[0][lane] = bsx;
+    L.bs[1][lane] = bsy;
+    L.bs[2][lane] = bsz;
+    L.off[0][lane] = ox;
+    L.off[1][lane] = oy;
+    L.off[2][lane] = oz;
+    if (A.use_fp32) {
+      L.off32[0][lane] = (h32[0] * (float)bsx + h32[1] * (float)bsy) + h32[2] * (float)bsz;
+      L.off32[1][lane] = (h32[3] * (float)bsx + h32[4] * (float)bsy) + h32[5] * (float)bsz;
+      L.off32[2][lane] = (h32[6] * (float)bsx + h32[7] * (float)bsy) + h32[8] * (float)bsz;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int total = __shfl(incl, 63, 64);
+    for (int c0 = 0; c0 < total; c0 += 64) {
+      const int idx = c0 + lane;
+      const bool valid = idx < total;
+      int lo = 0;
+      if (valid) {
+        int hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (L.incl[mid] > idx)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+      }
+      const int excl = lo > 0 ? L.incl[lo - 1] : 0;
+      const int j = L.start[lo] + (idx - excl);
+      const int sbx = L.bs[0][lo], sby = L.bs[1][lo], sbz = L.bs[2][lo];
+      const bool shifted = (sbx | sby | sbz) != 0;
+      int qs = A.k;
+      bool rule = false;
+      double d2 = INFINITY;
+      if (valid) {
+        if (A.use_fp32) {
+          const float4 c = A.spos32[j];
+          const int oj = __float_as_int(c.w);
+          rule = oi < oj || (oi == oj && shifted);
+          const float dx = (c.x - pi32.x) + L.off32[0][lo];
+          const float dy = (c.y - pi32.y) + L.off32[1][lo];
+          const float dz = (c.z - pi32.z) + L.off32[2][lo];
+          const float e2 = (dx * dx + dy * dy) + dz * dz;
+          if (rule && e2 <= rej32) {
+            // certain accept below lo_q, certain reject above hi_q, FP64
+            // recheck of the exact reference expression inside any band
+            int q32 = 0;
+            bool in_band = false;
+            for (int q = 0; q < A.k; ++q) {
+              const float2 bq = band[q];
+              q32 += e2 > bq.y;
+              in_band |= (e2 >= bq.x) && (e2 <= bq.y);
+            }
+            if (in_band || A.dist) {
+              const Pos64 c64 = A.spos64[j];
+              const double ddx = (c64.x - pi.x) + L.off[0][lo];
+              const double ddy = (c64.y - pi.y) + L.off[1][lo];
+              const double ddz = (c64.z - pi.z) + L.off[2][lo];
+              d2 = norm2(ddx, ddy, ddz);
+              qs = first_list(d2, A.rc2, A.k);
+            } else {
+              qs = q32;
+            }
+          }
+        } else {
+          const Pos64 c = A.spos64[j];
+          const int oj = (int)c.w;
+          rule = oi < oj || (oi == oj && shifted);
+          // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
+          const double dx = (c.x - pi.x) + L.off[0][lo];
+          const double dy = (c.y - pi.y) + L.off[1][lo];
+          const double dz = (c.z - pi.z) + L.off[2][lo];
+          d2 = norm2(dx, dy, dz);
+          if (rule) qs = first_list(d2, A.rc2, A.k);
+        }
+      }
+      const bool hit = valid && rule && qs < A.k;
+      if (__ballot(hit) == 0) continue;
+      int sx = 0, sy = 0, sz = 0;
+      int64_t gj = 0;
+      if (FILL && hit) {
+        // search.rs:695-697: S = s_j - s_i + bs
+        const int4 sj = A.sidx[j];
+        sx = sj.x - si.x + sbx;
+        sy = sj.y - si.y + sby;
+        sz = sj.z - si.z + sbz;
+        gj = A.global_index ? a0 + sj.w : (int64_t)sj.w;
+      }
+      emit_step<FILL>(A, hit, qs, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+}  // namespace
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
+  __shared__ WaveLds s_lds[kWaves];
+  const int w = threadIdx.x >> 6;
+  const int64_t u = (int64_t)blockIdx.x * kWaves + w;
+  if (u >= A.n) return;
+  const int s = uniform_i32(find_sys(A.sys_start, A.n_sys, u));
+  const SysDesc* d = A.sys + s;
+  const int mode = uniform_i32(d->mode);
+  int64_t cur[kGroup];
+#pragma unroll
+  for (int g = 0; g < kGroup; ++g)
+    cur[g] = (FILL && g < A.q_cnt) ? A.offsets[(int64_t)(A.q_lo + g) * A.n + u] : 0;
+  if (mode == kCellList)
+    unit_celllist<FILL>(A, d, s, u, s_lds[w], cur);
+  else
+    unit_bruteforce<FILL>(A, d, u, mode, cur);
+  if (!FILL && lane_id() == 0) {
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g)
+      if (g < A.q_cnt) A.counts[(int64_t)(A.q_lo + g) * A.n + u] = (uint32_t)cur[g];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of u32 counts into int64 offsets, decoupled look-back.
+// bounds[q] = offset of element q*seg (q < nseg), bounds[nseg] = total.
+__global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
+                                              int64_t* __restrict__ out, int64_t len,
+                                              int64_t seg, int nseg, int64_t* __restrict__ bounds,
+                                              uint64_t* __restrict__ status,
+                                              uint32_t* __restrict__ ticket) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_w[4];
+  __shared__ uint64_t s_excl;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int64_t e0 = (int64_t)tile * kRadixTile + 8 * tid;
+  uint64_t v[8];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    v[i] = (e0 + i < len) ? (uint64_t)in[e0 + i] : 0ull;
+    sum += v[i];
+  }
+  const uint64_t winc = wave_incl_scan(sum);
+  if (lane == 63) s_w[w] = winc;
+  __syncthreads();
+  uint64_t pre = winc - sum;
+  for (int i = 0; i < w; ++i) pre += s_w[i];
+  const uint64_t tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  if (tid == 0) {
+    uint64_t* st = status + tile;
+    uint64_t excl = 0;
+    if (tile == 0) {
+      atomic_store_agent(st, kFlagPre64 | tot);
+    } else {
+      atomic_store_agent(st, kFlagAgg64 | tot);
+      int64_t t = (int64_t)tile - 1;
+      while (true) {
+        const uint64_t x = atomic_load_agent(status + t);
+        const uint64_t flag = x & ~kValMask64;
+        if (flag == 0) continue;
+        excl += x & kValMask64;
+        if (flag == kFlagPre64) break;
+        --t;
+      }
+      atomic_store_agent(st, kFlagPre64 | (excl + tot));
+    }
+    s_excl = excl;
+  }
+  __syncthreads();
+  uint64_t run = s_excl + pre;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int64_t e = e0 + i;
+    if (e < len) {
+      out[e] = (int64_t)run;
+      if (e % seg == 0) bounds[e / seg] = (int64_t)run;
+      run += v[i];
+      if (e == len - 1) {
+        out[len] = (int64_t)run;
+        bounds[nseg] = (int64_t)run;
+      }
+    }
+  }
+}
+
+// Per-system edge ranges: ranges[(s*k + q)*2 + {0,1}].
+__global__ void k_sys_ranges(const int64_t* __restrict__ off, const SysDesc* __restrict__ sys,
+                             int n_sys, int k, int64_t n, int64_t* __restrict__ ranges) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_sys * k) return;
+  const int s = (int)(t / k), q = (int)(t % k);
+  const int64_t a0 = sys[s].atom_start, a1 = a0 + sys[s].n_atoms;
+  ranges[2 * t] = off[(int64_t)q * n + a0];
+  ranges[2 * t + 1] = off[(int64_t)q * n + a1];
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
+                       const Params& p, hipStream_t st) {
+  if (p.n_sys <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_plan, dim3(p.n_sys), dim3(256), 0, st, pos, sys, band, rc2, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
+                      int64_t n, uint32_t* keys, uint32_t* vals, uint32_t* hist, int passes,
+                      hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(n, 256)), dim3(256), 0, st, pos, sys, sys_start, n_sys,
+                     n, keys, vals, hist, passes);
+  return hipGetLastError();
+}
+
+hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                             uint32_t* vout, int64_t n, int shift, const uint32_t* hist,
+                             uint32_t* status, uint32_t* ticket, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_radix_pass, dim3((unsigned)radix_tiles(n)), dim3(256), 0, st, kin, vin,
+                     kout, vout, n, shift, hist, status, ticket);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const double* pos,
+                           const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,
+                           int64_t total_bins, int32_t* cell_start, Pos64* spos64,
+                           float4* spos32, int4* sidx, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, 256)), dim3(256), 0, st, keys, vals, pos, sys,
+                     sys_start, n_sys, n, total_bins, cell_start, spos64, spos32, sidx);
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st) {
+  if (a.n <= 0) return hipSuccess;
+  const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
+  if (fill)
+    hipLaunchKernelGGL(k_pairs<true>, g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL(k_pairs<false>, g, b, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
+                       int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st) {
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan, dim3((unsigned)radix_tiles(len)), dim3(256), 0, st, in, out, len,
+                     seg, nseg, bounds, status, ticket);
+  return hipGetLastError();
+}
+
+hipError_t launch_sys_ranges(const int64_t* offsets, const SysDesc* sys, int n_sys, int k,
+                             int64_t n, int64_t* ranges, hipStream_t st) {
+  const int64_t t = (int64_t)n_sys * k;
+  if (t <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sys_ranges, dim3(grid_for(t, 256)), dim3(256), 0, st, offsets, sys, n_sys,
+                     k, n, ranges);
+  return hipGetLastError();
+}
+
+}  // namespace nl

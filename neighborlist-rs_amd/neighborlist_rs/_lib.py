@@ -1,0 +1,87 @@
+"""ctypes binding of libnlmi355 (include/nl_api.h, include/nl_engine_ext.h).
+
+The shared library is built in-tree by neighborlist-rs_amd/build.py.  There
+is no CPU fallback: if the library is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnlmi355.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libnlmi355.so not found at {LIB_PATH}: build it with "
+        "`python neighborlist-rs_amd/build.py` (the engine has no CPU fallback)")
+
+lib = C.CDLL(LIB_PATH)
+
+dp = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+fp = C.POINTER(C.c_float)
+res_p = C.c_void_p
+res_pp = C.POINTER(C.c_void_p)
+
+_SIGS = {
+    # nl_api.h
+    "nl_search_single": (C.c_int, [dp, C.c_size_t, dp, u8p, C.c_double, C.c_int, res_pp]),
+    "nl_search_single_multi": (C.c_int, [dp, C.c_size_t, dp, u8p, dp, C.c_size_t, C.c_int,
+                                         res_pp]),
+    "nl_search_batch": (C.c_int, [dp, C.c_size_t, i32p, C.c_size_t, dp, u8p, u8p, C.c_double,
+                                  C.c_int, res_pp]),
+    "nl_search_batch_multi": (C.c_int, [dp, C.c_size_t, i32p, C.c_size_t, dp, u8p, u8p, dp,
+                                        C.c_size_t, C.c_int, res_pp]),
+    "nl_result_num_lists": (C.c_size_t, [res_p]),
+    "nl_result_num_edges": (C.c_int64, [res_p, C.c_size_t]),
+    "nl_result_copy": (C.c_int, [res_p, C.c_size_t, i64p, i64p, i32p]),
+    "nl_result_device": (C.c_int, [res_p, C.c_size_t, C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_void_p)]),
+    "nl_result_free": (None, [res_p]),
+    "nl_get_brute_force_threshold": (C.c_size_t, []),
+    "nl_set_brute_force_threshold": (None, [C.c_size_t]),
+    "nl_get_parallel_threshold": (C.c_size_t, []),
+    "nl_set_parallel_threshold": (None, [C.c_size_t]),
+    "nl_get_stack_threshold": (C.c_size_t, []),
+    "nl_set_stack_threshold": (None, [C.c_size_t]),
+    "nl_last_error": (C.c_char_p, []),
+    "nl_version": (C.c_char_p, []),
+    # nl_engine_ext.h
+    "nl_search_batch_device": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p,
+                                         dp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, res_pp]),
+    "nl_search_systems": (C.c_int, [dp, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp,
+                                    C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    res_pp]),
+    "nl_result_system_ranges": (C.c_int, [res_p, i64p, i32p]),
+    "nl_result_copy_dist": (C.c_int, [res_p, C.c_size_t, dp]),
+    "nl_set_timing": (None, [C.c_int]),
+    "nl_last_timing": (C.c_int, [fp, fp, fp, fp, i64p, i64p]),
+    "nl_set_cell_div": (None, [C.c_int]),
+    "nl_get_cell_div": (C.c_int, []),
+    "nl_set_fp32_filter": (None, [C.c_int]),
+    "nl_get_fp32_filter": (C.c_int, []),
+    "nl_cell_inverse": (C.c_int, [dp, dp]),
+    "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
+    "nl_device_malloc": (C.c_void_p, [C.c_size_t]),
+    "nl_device_free": (C.c_int, [C.c_void_p]),
+    "nl_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "nl_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "nl_device_synchronize": (C.c_int, []),
+    "nl_set_device": (C.c_int, [C.c_int]),
+    "nl_device_count": (C.c_int, []),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def last_error() -> str:
+    return lib.nl_last_error().decode()

@@ -1,0 +1,198 @@
+"""GPU parity: the HIP engine against the CPU oracle, through the C ABI.
+
+Bit-exact bar: after the canonical sort on (i, j, S), edge_index and shift
+arrays are identical to the oracle's (which restates the reference and is
+pinned by the reference's own known answers, see test_oracle_reference.py).
+"""
+import numpy as np
+import pytest
+
+from conftest import as_tuple, assert_same_edges
+from xyzio import read_xyz
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(*a):
+    print("[case]", *a, flush=True)
+
+
+def _gpu_single(nr, pos, H, pbc, rc):
+    cell = None if H is None else nr.PyCell(np.asarray(H).tolist(), pbc)
+    return as_tuple(nr.build_neighborlists(cell, pos, rc))
+
+
+FIXTURES = [
+    ("si_bulk.xyz", 6.0), ("si_bulk.xyz", 14.0), ("isolated_100.xyz", 6.0),
+    ("isolated_100.xyz", 14.0), ("isolated_two_clusters.xyz", 6.0),
+    ("isolated_1000.xyz", 6.0), ("isolated_1000.xyz", 14.0), ("ethanol_1000.xyz", 6.0),
+    ("ethanol_1000.xyz", 14.0),
+]
+
+
+@pytest.mark.parametrize("name,rc", FIXTURES)
+def test_fixture_vs_oracle(nr, oracle, name, rc):
+    _print_case("fixture", name, rc)
+    pos, rows, pbc = read_xyz(name)
+    H = None if rows is None else rows.T
+    want = oracle.search_single(pos, H, pbc if rows is not None else None, rc)
+    got = _gpu_single(nr, pos, H, pbc, rc)
+    assert_same_edges(got, want, f"{name} rc={rc}")
+
+
+# counts the reference logged for its fixtures (benchmarks/benchmark_EPYC7R13.txt),
+# self-image (i == j) edges excluded as in comprehensive_benchmark.py:96-100
+LOGGED = [("ethanol_10000.xyz", 6.0, 568606), ("ethanol_10000.xyz", 14.0, 6807370),
+          ("isolated_10000.xyz", 6.0, 482454)]
+
+
+@pytest.mark.parametrize("name,rc,count", LOGGED)
+def test_large_fixture_logged_counts(nr, name, rc, count):
+    _print_case("logged", name, rc)
+    pos, rows, pbc = read_xyz(name)
+    H = None if rows is None else rows.T
+    ei, ej, s = _gpu_single(nr, pos, H, pbc, rc)
+    assert int((ei != ej).sum()) == count
+    assert np.all((ei < ej) | ((ei == ej) & np.any(s != 0, axis=1)))
+
+
+@pytest.mark.parametrize("cell_div", [1, 2, 3])
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_configs_vs_oracle(nr, oracle, cell_div, fp32):
+    from neighborlist_rs import workloads as W
+    from neighborlist_rs._lib import lib
+    lib.nl_set_cell_div(cell_div)
+    lib.nl_set_fp32_filter(fp32)
+    try:
+        cases = [("c1", *W.c1_isolated())]
+        p, H, rcs = W.c2_ethanol()
+        cases += [("c2", p, H, rc) for rc in rcs]
+        p, H, rcs = W.c3_triclinic()
+        cases += [("c3", p, H, 6.0)]
+        for name, p, H, rc in cases:
+            _print_case("config", name, rc, "div", cell_div, "fp32", fp32)
+            want = oracle.search_single(p, H, None if H is None else [1, 1, 1], rc)
+            got = _gpu_single(nr, p, H, [True] * 3, rc)
+            assert_same_edges(got, want, f"{name} rc={rc} div={cell_div} fp32={fp32}")
+    finally:
+        lib.nl_set_cell_div(2)
+        lib.nl_set_fp32_filter(0)
+
+
+def test_c3_rc14_vs_oracle(nr, oracle):
+    from neighborlist_rs import workloads as W
+    p, H, _ = W.c3_triclinic()
+    _print_case("config c3 rc14")
+    want = oracle.search_single(p, H, [1, 1, 1], 14.0)
+    got = _gpu_single(nr, p, H, [True] * 3, 14.0)
+    assert_same_edges(got, want, "c3 rc=14")
+
+
+@pytest.mark.parametrize("disjoint", [False, True])
+def test_multi_vs_oracle(nr, oracle, disjoint):
+    from neighborlist_rs import workloads as W
+    p, H, _ = W.c2_ethanol()
+    cut = [14.0, 5.0, 6.0]
+    _print_case("multi c2", disjoint)
+    want = oracle.search_single_multi(p, H, [1, 1, 1], cut, disjoint)
+    got = nr.build_neighborlists_multi(nr.PyCell(H.tolist()), p, cut, disjoint=disjoint)
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"list {k}")
+
+
+@pytest.mark.parametrize("disjoint", [False, True])
+def test_ragged_batch_multi_vs_oracle(nr, oracle, disjoint):
+    from neighborlist_rs import workloads as W
+    p, sizes, cells, cut = W.c4_ragged_batch(n_systems=64, nmax=1200)
+    batch, rows = W.batch_arrays(sizes, cells)
+    _print_case("batch multi", len(sizes), disjoint)
+    ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+    want = oracle.search_batch_multi(p, batch, ocells, cut, disjoint)
+    got = nr.build_neighborlists_batch_multi(p, batch, rows, list(cut), disjoint=disjoint)
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"list {k}")
+
+
+def test_ragged_batch_single_vs_oracle(nr, oracle):
+    from neighborlist_rs import workloads as W
+    p, sizes, cells, _ = W.c4_ragged_batch(n_systems=48, nmax=1500, seed=11)
+    batch, rows = W.batch_arrays(sizes, cells)
+    _print_case("batch single", len(sizes))
+    ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+    want = oracle.search_batch(p, batch, ocells, 6.0)
+    got = as_tuple(nr.build_neighborlists_batch(p, batch, rows, 6.0))
+    assert_same_edges(got, want, "batch rc=6")
+
+
+def test_bruteforce_row_order_matches_reference(nr, oracle):
+    """Brute-force systems come out in the reference's row-major order."""
+    rng = np.random.default_rng(7)
+    p = rng.uniform(0, 12, size=(300, 3))
+    _print_case("bf order")
+    want = oracle.search_single(p, None, None, 3.0)
+    got = _gpu_single(nr, p, None, None, 3.0)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("thr", [0, 1000, 100000])
+def test_dispatch_threshold(nr, oracle, thr):
+    """brute_force_threshold keeps its dispatch meaning (config.rs:8, single.rs:64)."""
+    rng = np.random.default_rng(8)
+    H = np.diag([14.0, 15.0, 16.0])
+    p = rng.uniform(0, 14, size=(500, 3))
+    old = nr.get_brute_force_threshold()
+    nr.set_brute_force_threshold(thr)
+    oracle.set_threshold("brute_force", thr)
+    try:
+        _print_case("threshold", thr)
+        for pbc in ([True, True, True], [True, False, True], [False, False, False]):
+            want = oracle.search_single(p, H, pbc, 5.0)
+            got = _gpu_single(nr, p, H, pbc, 5.0)
+            assert_same_edges(got, want, f"thr={thr} pbc={pbc}")
+    finally:
+        nr.set_brute_force_threshold(old)
+        oracle.set_threshold("brute_force", old)
+
+
+def test_small_box_many_images(nr, oracle):
+    """rc far beyond the box: many periodic images (search.rs:1862-1882)."""
+    _print_case("many images")
+    for H, p, rc in [(np.eye(3), np.array([[0.5, 0.5, 0.5]]), 10.0),
+                     (np.eye(3) * 5.0, np.array([[0.0, 0.0, 0.0]]), 6.0),
+                     (np.diag([4.1, 4.4, 4.7]), np.random.default_rng(3).uniform(0, 4, (8, 3)),
+                      14.0)]:
+        want = oracle.search_single(p, H, [1, 1, 1], rc)
+        got = _gpu_single(nr, p, H, [True] * 3, rc)
+        assert_same_edges(got, want, f"rc={rc}")
+
+
+def test_distances_fp64(nr, oracle):
+    """FP64 |r_ij| emitted with the edges equals a float64 recomputation."""
+    import ctypes as C
+    from neighborlist_rs import workloads as W
+    from neighborlist_rs._lib import lib, dp, i64p, i32p
+    p, H, _ = W.c2_ethanol()
+    sizes = np.array([p.shape[0]], np.int64)
+    cut = np.array([6.0])
+    has = np.ones(1, np.uint8)
+    pb = np.ones(3, np.uint8)
+    Hf = np.ascontiguousarray(H.reshape(9))
+    r = C.c_void_p()
+    assert lib.nl_search_systems(p.ctypes.data_as(dp), p.shape[0], sizes.ctypes.data_as(i64p), 1,
+                                 Hf.ctypes.data_as(dp), has.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 pb.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 cut.ctypes.data_as(dp), 1, 0, 0, 1, 0, 1, C.byref(r)) == 0
+    m = lib.nl_result_num_edges(r, 0)
+    ei = np.empty(m, np.int64)
+    ej = np.empty(m, np.int64)
+    s = np.empty((m, 3), np.int32)
+    d = np.empty(m)
+    assert lib.nl_result_copy(r, 0, ei.ctypes.data_as(i64p), ej.ctypes.data_as(i64p),
+                              s.ctypes.data_as(i32p)) == 0
+    assert lib.nl_result_copy_dist(r, 0, d.ctypes.data_as(dp)) == 0
+    lib.nl_result_free(r)
+    rij = p[ej] + s @ H.T - p[ei]
+    ref = np.sqrt((rij ** 2).sum(1))
+    np.testing.assert_allclose(d, ref, rtol=0, atol=1e-12)
+    assert np.all(d < 6.0)
