@@ -28,8 +28,8 @@ Result::~Result() {
 
 // scratch slots
 enum Slot {
-  kPos, kSys, kStart, kRc2, kBand, kKeysA, kValsA, kKeysB, kValsB, kHist, kRStatus, kTickets,
-  kCellStart, kSpos64, kSpos32, kSidx, kCounts, kOffsets, kSStatus, kBounds, kRanges, kSlots
+  kPos, kSys, kStart, kRc2, kBand, kKeysA, kValsA, kKeysB, kValsB, kZero, kCellStart, kSpos64,
+  kSpos32, kSidx, kCounts, kOffsets, kBounds, kRanges, kSlots
 };
 
 Engine::Engine(int device) : device_(device), scratch_(kSlots) {
@@ -62,6 +62,41 @@ void* Engine::scratch(int slot, size_t bytes) {
     s.cap = cap;
   }
   return s.p;
+}
+
+void* Engine::pinned(size_t bytes) {
+  if (pinned_cap_ < bytes) {
+    if (pinned_) (void)hipHostFree(pinned_);
+    size_t cap = std::max(bytes, pinned_cap_ * 2);
+    if (hipHostMalloc(&pinned_, cap, hipHostMallocDefault) != hipSuccess) {
+      pinned_ = nullptr;
+      pinned_cap_ = 0;
+      return nullptr;
+    }
+    pinned_cap_ = cap;
+  }
+  return pinned_;
+}
+
+int Engine::alloc_outputs(Result* res, int64_t edges, bool dist) {
+  if (res->edges) dev_release(res->edges, res->edges_bytes);
+  if (res->shifts) dev_release(res->shifts, res->shifts_bytes);
+  if (res->dist) dev_release(res->dist, res->dist_bytes);
+  res->edges = res->shifts = res->dist = nullptr;
+  const int64_t m = std::max<int64_t>(edges, 1);
+  res->edges_bytes = sizeof(int64_t) * 2 * m;
+  res->shifts_bytes = sizeof(int32_t) * 3 * m;
+  res->edges = dev_alloc(res->edges_bytes);
+  res->shifts = dev_alloc(res->shifts_bytes);
+  if (dist) {
+    res->dist_bytes = sizeof(double) * m;
+    res->dist = dev_alloc(res->dist_bytes);
+  }
+  if (!res->edges || !res->shifts || (dist && !res->dist)) {
+    set_error("out of device memory for the edge lists");
+    return 1;
+  }
+  return 0;
 }
 
 void* Engine::dev_alloc(size_t bytes) {
@@ -192,38 +227,52 @@ int Engine::run(const Request& req, Result** out) {
   uint32_t* valsA = (uint32_t*)scratch(kValsA, 4 * n);
   uint32_t* keysB = (uint32_t*)scratch(kKeysB, 4 * n);
   uint32_t* valsB = (uint32_t*)scratch(kValsB, 4 * n);
-  uint32_t* hist = (uint32_t*)scratch(kHist, 4 * 256 * passes);
-  uint32_t* rstatus = (uint32_t*)scratch(kRStatus, 4 * 256 * rtiles * passes);
-  uint32_t* tickets = (uint32_t*)scratch(kTickets, 4 * 8);
+  // one zero-initialised block: radix histograms, radix look-back status,
+  // tile tickets, scan look-back status (64-bit aligned first)
+  const int64_t z_sstatus = 2 * stiles;             // u32 words
+  const int64_t z_hist = 256 * passes;
+  const int64_t z_rstatus = 256 * rtiles * passes;
+  const int64_t z_tickets = 8;
+  const int64_t z_words = z_sstatus + z_hist + z_rstatus + z_tickets;
+  uint32_t* zero = (uint32_t*)scratch(kZero, 4 * z_words);
   int32_t* cell_start = (int32_t*)scratch(kCellStart, 4 * (bin_total + 1));
   Pos64* spos64 = (Pos64*)scratch(kSpos64, sizeof(Pos64) * n);
   float4* spos32 = (float4*)scratch(kSpos32, sizeof(float4) * n);
   int4* sidx = (int4*)scratch(kSidx, sizeof(int4) * n);
   uint32_t* counts = (uint32_t*)scratch(kCounts, 4 * scan_len);
   int64_t* offsets = (int64_t*)scratch(kOffsets, 8 * (scan_len + 1));
-  uint64_t* sstatus = (uint64_t*)scratch(kSStatus, 8 * stiles);
   int64_t* bounds = (int64_t*)scratch(kBounds, 8 * (K + 1));
+  const size_t pin_desc = sizeof(SysDesc) * S, pin_start = 8 * (S + 1), pin_rc2 = 8 * K;
+  const size_t pin_bounds = 8 * (K + 1);
+  char* pin = (char*)pinned(pin_desc + pin_start + pin_rc2 + pin_bounds + 64);
   if (!d_pos || !d_sys || !d_start || !d_rc2 || !d_band || !keysA || !valsA || !keysB || !valsB ||
-      !hist || !rstatus || !tickets || !cell_start || !spos64 || !spos32 || !sidx || !counts ||
-      !offsets || !sstatus || !bounds) {
+      !zero || !cell_start || !spos64 || !spos32 || !sidx || !counts || !offsets || !bounds ||
+      !pin) {
     set_error("out of device memory");
     return 1;
   }
+  uint64_t* sstatus = (uint64_t*)zero;
+  uint32_t* hist = zero + z_sstatus;
+  uint32_t* rstatus = hist + z_hist;
+  uint32_t* tickets = rstatus + z_rstatus;
+  char* h_desc = pin;
+  char* h_start = h_desc + pin_desc;
+  char* h_rc2 = h_start + pin_start;
+  int64_t* h_bounds = (int64_t*)(h_rc2 + pin_rc2);
+  std::memcpy(h_desc, desc.data(), pin_desc);
+  std::memcpy(h_start, req.starts.data(), pin_start);
+  std::memcpy(h_rc2, rc2.data(), pin_rc2);
 
   hipStream_t st = stream_;
   if (!req.pos_device)
     NL_CHECK(hipMemcpyAsync(d_pos, req.pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_sys, desc.data(), sizeof(SysDesc) * S, hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_start, req.starts.data(), sizeof(int64_t) * (S + 1),
-                          hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_rc2, rc2.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+  NL_CHECK(hipMemcpyAsync(d_sys, h_desc, pin_desc, hipMemcpyHostToDevice, st));
+  NL_CHECK(hipMemcpyAsync(d_start, h_start, pin_start, hipMemcpyHostToDevice, st));
+  NL_CHECK(hipMemcpyAsync(d_rc2, h_rc2, pin_rc2, hipMemcpyHostToDevice, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
 
   // build: plan, bin, radix sort, finalize
-  NL_CHECK(launch_fill_u32(hist, 256 * passes, 0, st));
-  NL_CHECK(launch_fill_u32(rstatus, 256 * rtiles * passes, 0, st));
-  NL_CHECK(launch_fill_u32(tickets, 8, 0, st));
-  NL_CHECK(launch_fill_u32((uint32_t*)sstatus, 2 * stiles, 0, st));
+  NL_CHECK(launch_fill_u32(zero, z_words, 0, st));
   NL_CHECK(launch_plan(d_pos, d_sys, d_band, d_rc2, prm, st));
   NL_CHECK(launch_bin(d_pos, d_sys, d_start, S, n, keysA, valsA, hist, passes, st));
   uint32_t *kin = keysA, *vin = valsA, *kout = keysB, *vout = valsB;
@@ -264,36 +313,41 @@ int Engine::run(const Request& req, Result** out) {
   }
   NL_CHECK(launch_scan(counts, offsets, scan_len, n, K, bounds, sstatus, tickets + 4, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[2], st));
-  std::vector<int64_t> hb(K + 1);
-  NL_CHECK(hipMemcpyAsync(hb.data(), bounds, 8 * (K + 1), hipMemcpyDeviceToHost, st));
-  NL_CHECK(hipStreamSynchronize(st));
 
-  const int64_t total = hb[K];
-  res->total = total;
-  res->edges_bytes = sizeof(int64_t) * 2 * std::max<int64_t>(total, 1);
-  res->shifts_bytes = sizeof(int32_t) * 3 * std::max<int64_t>(total, 1);
-  res->edges = dev_alloc(res->edges_bytes);
-  res->shifts = dev_alloc(res->shifts_bytes);
-  if (req.want_dist) {
-    res->dist_bytes = sizeof(double) * std::max<int64_t>(total, 1);
-    res->dist = dev_alloc(res->dist_bytes);
-  }
-  if (!res->edges || !res->shifts || (req.want_dist && !res->dist)) {
-    set_error("out of device memory for the edge lists");
-    return 1;
-  }
-  A.edges = (int64_t*)res->edges;
-  A.shifts = (int32_t*)res->shifts;
-  A.dist = (double*)res->dist;
-  if (timing_) NL_CHECK(hipEventRecord(ev_[3], st));
-  if (total > 0) {
+  // Fill straight away when an estimate of the output size exists (same
+  // request shape as last time); the kernel writes nothing if it is short.
+  const int64_t sig[4] = {n, S, K, (int64_t)req.multi * 2 + req.disjoint};
+  const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
+  int64_t cap = have_est ? last_total_ + last_total_ / 8 + 4096 : -1;
+  auto launch_fill = [&](int64_t capacity) -> int {
+    A.edges = (int64_t*)res->edges;
+    A.shifts = (int32_t*)res->shifts;
+    A.dist = (double*)res->dist;
+    A.capacity = capacity;
+    if (timing_) NL_CHECK(hipEventRecord(ev_[3], st));
     for (int q0 = 0; q0 < K; q0 += kGroup) {
       A.q_lo = q0;
       A.q_cnt = std::min(kGroup, K - q0);
       NL_CHECK(launch_pairs(true, A, st));
     }
+    if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
+    return 0;
+  };
+  if (have_est) {
+    if (alloc_outputs(res.get(), cap, req.want_dist)) return 1;
+    if (launch_fill(cap)) return 1;
   }
-  if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
+  NL_CHECK(hipMemcpyAsync(h_bounds, bounds, pin_bounds, hipMemcpyDeviceToHost, st));
+  NL_CHECK(hipStreamSynchronize(st));
+  std::vector<int64_t> hb(h_bounds, h_bounds + K + 1);
+  const int64_t total = hb[K];
+  res->total = total;
+  if (!have_est || total > cap) {
+    if (alloc_outputs(res.get(), total, req.want_dist)) return 1;
+    if (total > 0 && launch_fill(total)) return 1;
+  }
+  for (int q = 0; q < 4; ++q) last_sig_[q] = sig[q];
+  last_total_ = total;
   for (int q = 0; q < K; ++q) {
     const int r = (int)(std::find(order.begin(), order.end(), q) - order.begin());
     res->base[q] = hb[r];
@@ -329,9 +383,12 @@ int Engine::run(const Request& req, Result** out) {
     times_.build = t;
     (void)hipEventElapsedTime(&t, ev_[1], ev_[2]);
     times_.count = t;
-    (void)hipEventElapsedTime(&t, ev_[3], ev_[4]);
-    times_.fill = t;
-    (void)hipEventElapsedTime(&t, ev_[0], ev_[4]);
+    times_.fill = 0;
+    if (total > 0) {
+      (void)hipEventElapsedTime(&t, ev_[3], ev_[4]);
+      times_.fill = t;
+    }
+    (void)hipEventElapsedTime(&t, ev_[0], total > 0 ? ev_[4] : ev_[2]);
     times_.total = t;
     times_.units = n;
     times_.edges = total;

@@ -96,7 +96,9 @@ class Engine {
     size_t cap = 0;
   };
   void* scratch(int slot, size_t bytes);
+  void* pinned(size_t bytes);
   int fail(hipError_t e, const char* where);
+  int alloc_outputs(Result* res, int64_t edges, bool dist);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -107,6 +109,12 @@ class Engine {
   hipEvent_t ev_[6] = {};
   bool timing_ = false;
   StageTimes times_;
+  void* pinned_ = nullptr;  // host staging for small transfers (hipHostMalloc)
+  size_t pinned_cap_ = 0;
+  // output-size estimate of the last request shape: lets the fill pass be
+  // launched without waiting for the counts
+  int64_t last_sig_[4] = {-1, -1, -1, -1};
+  int64_t last_total_ = -1;
 };
 
 Engine* engine_for_current_device();

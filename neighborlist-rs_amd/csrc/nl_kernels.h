@@ -27,6 +27,7 @@ struct PairArgs {
   int32_t* shifts;           // list q: 3 per edge at 3 e
   double* dist;              // optional |r_ij| per edge (FP64), list q at e
   int64_t n;
+  int64_t capacity;          // edges the output buffers hold (fill pass)
   int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index, pad;
 };
 

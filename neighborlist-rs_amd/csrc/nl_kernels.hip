@@ -384,8 +384,9 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Finalize: one thread per sorted slot p.  Cell offsets from key boundaries;
-// wrapped position and wrap shift of the atom, as CellList::build computes
+// Finalize: thread t writes the offset of bin t (lower bound of t in the
+// sorted keys, for t <= total_bins) and, for t < n, the sorted slot t: the
+// wrapped position and wrap shift of its atom, as CellList::build computes
 // them (search.rs:189-204): s = -floor(H^-1 r), wrapped = r - H (-s).
 __global__ __launch_bounds__(256) void k_finalize(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
@@ -393,13 +394,20 @@ __global__ __launch_bounds__(256) void k_finalize(
     const int64_t* __restrict__ sys_start, int n_sys, int64_t n, int64_t total_bins,
     int32_t* __restrict__ cell_start, Pos64* __restrict__ spos64, float4* __restrict__ spos32,
     int4* __restrict__ sidx) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int64_t key = keys[p];
-  const int64_t prev = p > 0 ? (int64_t)keys[p - 1] : -1;
-  for (int64_t b = prev + 1; b <= key; ++b) cell_start[b] = (int32_t)p;
-  if (p == n - 1)
-    for (int64_t b = key + 1; b <= total_bins; ++b) cell_start[b] = (int32_t)n;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= total_bins) {
+    int64_t lo = 0, hi = n;  // first slot with key >= t
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)keys[mid] < t)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    cell_start[t] = (int32_t)lo;
+  }
+  if (t >= n) return;
+  const int64_t p = t;
   const int64_t a = vals[p];
   const int s = find_sys(sys_start, n_sys, a);
   const SysDesc* d = sys + s;
@@ -517,6 +525,17 @@ __device__ void unit_bruteforce(const PairArgs& A, const SysDesc* d, int64_t u, 
   }
 }
 
+// Cell-list unit: one sorted slot p.  Half stencil: every unordered
+// (atom, atom, image) triple is examined once, from the slot whose stencil
+// offset to the other is lexicographically positive (or, inside the home
+// bin with no image shift, from the lower slot).  The reference examines it
+// from the lower original index with d = (w_j - w_i) + H bs; from the other
+// end the same pair gives (w_i - w_j) + H (-bs), which is the exact negation
+// of every rounded term (IEEE subtraction and products are sign-symmetric),
+// so d^2 and therefore every decision are bit-identical.  Edges are emitted
+// in the reference's orientation (lower original index first, S from it);
+// a self-image (same atom, bs != 0) is emitted for both S and -S, as the
+// reference emits both (search.rs:549, Q3).
 template <bool FILL>
 __device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_t p,
                               WaveLds& L, int64_t* cur) {
@@ -531,15 +550,15 @@ __device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_
   const int4 si = A.sidx[p];
   const int oi = si.w;
   const float4 pi32 = A.spos32[p];
-  const int64_t gi = A.global_index ? a0 + oi : (int64_t)oi;
   const double h0 = d->h[0], h1 = d->h[1], h2 = d->h[2], h3 = d->h[3], h4 = d->h[4],
                h5 = d->h[5], h6 = d->h[6], h7 = d->h[7], h8 = d->h[8];
   const float* h32 = d->h32;
   const int wx = 2 * nsx + 1, wy = 2 * nsy + 1, wz = 2 * nsz + 1;
   const int T = wx * wy * wz;
+  const int tc = (nsx * wy + nsy) * wz + nsz;  // the (0,0,0) offset
   const float2* band = A.band + (int64_t)s * A.k;
   const float rej32 = band[A.k - 1].y;  // beyond the largest band: certain miss
-  for (int t0 = 0; t0 < T; t0 += 64) {
+  for (int t0 = tc; t0 < T; t0 += 64) {
     const int t = t0 + lane;
     int len = 0, st = 0, bsx = 0, bsy = 0, bsz = 0;
     double ox = 0, oy = 0, oz = 0;
@@ -554,7 +573,9 @@ __device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_
       div_mod(bz + dz, nbz, &cz, &bsz);
       const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
       st = A.cell_start[bin];
-      len = A.cell_start[bin + 1] - st;
+      const int en = A.cell_start[bin + 1];
+      if (t == tc) st = (int)p + 1;  // home bin, no image: slots after p only
+      len = en - st;
       // (ox + oy) + oz of search.rs:498 = H (bsx, bsy, bsz) in gemv order
       ox = (h0 * (double)bsx + h1 * (double)bsy) + h2 * (double)bsz;
       oy = (h3 * (double)bsx + h4 * (double)bsy) + h5 * (double)bsz;
@@ -574,39 +595,33 @@ __device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_
       L.off32[1][lane] = (h32[3] * (float)bsx + h32[4] * (float)bsy) + h32[5] * (float)bsz;
       L.off32[2][lane] = (h32[6] * (float)bsx + h32[7] * (float)bsy) + h32[8] * (float)bsz;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int total = __shfl(incl, 63, 64);
+    int seg0 = 0;  // first segment that still holds candidates at c0
     for (int c0 = 0; c0 < total; c0 += 64) {
+      while (L.incl[seg0] <= c0) ++seg0;  // uniform: every lane sees the same
       const int idx = c0 + lane;
       const bool valid = idx < total;
-      int lo = 0;
-      if (valid) {
-        int hi = 63;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (L.incl[mid] > idx)
-            hi = mid;
-          else
-            lo = mid + 1;
-        }
-      }
-      const int excl = lo > 0 ? L.incl[lo - 1] : 0;
-      const int j = L.start[lo] + (idx - excl);
-      const int sbx = L.bs[0][lo], sby = L.bs[1][lo], sbz = L.bs[2][lo];
-      const bool shifted = (sbx | sby | sbz) != 0;
+      int sg = seg0;
+      if (valid)
+        while (L.incl[sg] <= idx) ++sg;
+      const int excl = sg > 0 ? L.incl[sg - 1] : 0;
+      const int j = L.start[sg] + (idx - excl);
+      const int sbx = L.bs[0][sg], sby = L.bs[1][sg], sbz = L.bs[2][sg];
       int qs = A.k;
-      bool rule = false;
+      int oj = 0;
       double d2 = INFINITY;
       if (valid) {
         if (A.use_fp32) {
           const float4 c = A.spos32[j];
-          const int oj = __float_as_int(c.w);
-          rule = oi < oj || (oi == oj && shifted);
-          const float dx = (c.x - pi32.x) + L.off32[0][lo];
-          const float dy = (c.y - pi32.y) + L.off32[1][lo];
-          const float dz = (c.z - pi32.z) + L.off32[2][lo];
+          oj = __float_as_int(c.w);
+          const float dx = (c.x - pi32.x) + L.off32[0][sg];
+          const float dy = (c.y - pi32.y) + L.off32[1][sg];
+          const float dz = (c.z - pi32.z) + L.off32[2][sg];
           const float e2 = (dx * dx + dy * dy) + dz * dz;
-          if (rule && e2 <= rej32) {
+          if (e2 <= rej32) {
             // certain accept below lo_q, certain reject above hi_q, FP64
             // recheck of the exact reference expression inside any band
             int q32 = 0;
@@ -618,9 +633,9 @@ __device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_
             }
             if (in_band || A.dist) {
               const Pos64 c64 = A.spos64[j];
-              const double ddx = (c64.x - pi.x) + L.off[0][lo];
-              const double ddy = (c64.y - pi.y) + L.off[1][lo];
-              const double ddz = (c64.z - pi.z) + L.off[2][lo];
+              const double ddx = (c64.x - pi.x) + L.off[0][sg];
+              const double ddy = (c64.y - pi.y) + L.off[1][sg];
+              const double ddz = (c64.z - pi.z) + L.off[2][sg];
               d2 = norm2(ddx, ddy, ddz);
               qs = first_list(d2, A.rc2, A.k);
             } else {
@@ -629,29 +644,41 @@ __device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_
           }
         } else {
           const Pos64 c = A.spos64[j];
-          const int oj = (int)c.w;
-          rule = oi < oj || (oi == oj && shifted);
+          oj = (int)c.w;
           // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-          const double dx = (c.x - pi.x) + L.off[0][lo];
-          const double dy = (c.y - pi.y) + L.off[1][lo];
-          const double dz = (c.z - pi.z) + L.off[2][lo];
+          const double dx = (c.x - pi.x) + L.off[0][sg];
+          const double dy = (c.y - pi.y) + L.off[1][sg];
+          const double dz = (c.z - pi.z) + L.off[2][sg];
           d2 = norm2(dx, dy, dz);
-          if (rule) qs = first_list(d2, A.rc2, A.k);
+          qs = first_list(d2, A.rc2, A.k);
         }
       }
-      const bool hit = valid && rule && qs < A.k;
+      const bool hit = valid && qs < A.k;
       if (__ballot(hit) == 0) continue;
+      const bool self_img = hit && (oj == oi);
       int sx = 0, sy = 0, sz = 0;
-      int64_t gj = 0;
+      int64_t gi = 0, gj = 0;
       if (FILL && hit) {
-        // search.rs:695-697: S = s_j - s_i + bs
+        // search.rs:695-697: S = s_j - s_i + bs, oriented from the lower index
         const int4 sj = A.sidx[j];
         sx = sj.x - si.x + sbx;
         sy = sj.y - si.y + sby;
         sz = sj.z - si.z + sbz;
-        gj = A.global_index ? a0 + sj.w : (int64_t)sj.w;
+        int lo_i = oi, hi_i = oj;
+        if (oj < oi) {
+          lo_i = oj;
+          hi_i = oi;
+          sx = -sx;
+          sy = -sy;
+          sz = -sz;
+        }
+        gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
+        gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
       }
-      emit_step<FILL>(A, hit, qs, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
+      const double dd = A.dist ? sqrt(d2) : 0.0;
+      emit_step<FILL>(A, hit, qs, gi, gj, sx, sy, sz, dd, cur);
+      if (__ballot(self_img))
+        emit_step<FILL>(A, self_img, qs, gi, gj, -sx, -sy, -sz, dd, cur);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -665,6 +692,9 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   const int w = threadIdx.x >> 6;
   const int64_t u = (int64_t)blockIdx.x * kWaves + w;
   if (u >= A.n) return;
+  // fill pass launched before the host saw the counts: write nothing when
+  // the lists do not fit; the host re-runs the fill with exact buffers
+  if (FILL && A.bounds[A.k] > A.capacity) return;
   const int s = uniform_i32(find_sys(A.sys_start, A.n_sys, u));
   const SysDesc* d = A.sys + s;
   const int mode = uniform_i32(d->mode);
@@ -802,7 +832,8 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
                            int64_t total_bins, int32_t* cell_start, Pos64* spos64,
                            float4* spos32, int4* sidx, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, 256)), dim3(256), 0, st, keys, vals, pos, sys,
+  const int64_t threads = n > total_bins + 1 ? n : total_bins + 1;
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(threads, 256)), dim3(256), 0, st, keys, vals, pos, sys,
                      sys_start, n_sys, n, total_bins, cell_start, spos64, spos32, sidx);
   return hipGetLastError();
 }
