@@ -55,6 +55,11 @@ void nl_set_cell_div(int d);
 int nl_get_cell_div(void);
 void nl_set_fp32_filter(int on);
 int nl_get_fp32_filter(void);
+/* Work decomposition of the cell-list pair kernels: `group` home atoms of
+ * one bin per wavefront (1..64) and `parts` slices of the half stencil per
+ * home group (1..16).  Results never depend on them. */
+void nl_set_work_split(int group, int parts);
+int nl_get_work_split(int* group, int* parts);
 
 /* Host-side cell helpers with the reference's arithmetic (no GPU needed):
  * H^-1 by the cofactor formula of Cell::new (cell.rs:18-21; returns nonzero

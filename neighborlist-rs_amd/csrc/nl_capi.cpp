@@ -362,6 +362,15 @@ void nl_set_cell_div(int d) { nl::config().cell_div.store(d < 1 ? 1 : (d > 4 ? 4
 int nl_get_cell_div(void) { return nl::config().cell_div.load(); }
 void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }
 int nl_get_fp32_filter(void) { return nl::config().use_fp32.load(); }
+void nl_set_work_split(int group, int parts) {
+  nl::config().group.store(group < 1 ? 1 : (group > 64 ? 64 : group));
+  nl::config().parts.store(parts < 1 ? 1 : (parts > 16 ? 16 : parts));
+}
+int nl_get_work_split(int* group, int* parts) {
+  if (group) *group = nl::config().group.load();
+  if (parts) *parts = nl::config().parts.load();
+  return 0;
+}
 
 int nl_cell_inverse(const double* h, double* hinv) {
   if (!nl::inverse3(h, hinv)) {

@@ -28,7 +28,7 @@ Result::~Result() {
 
 // scratch slots
 enum Slot {
-  kPos, kSys, kStart, kRc2, kBand, kKeysA, kValsA, kKeysB, kValsB, kZero, kCellStart, kSpos64,
+  kPos, kSys, kStart, kItems, kRc2, kBand, kKeysA, kValsA, kKeysB, kValsB, kZero, kCellStart, kSpos64,
   kSpos32, kSidx, kCounts, kOffsets, kBounds, kRanges, kSlots
 };
 
@@ -199,7 +199,14 @@ int Engine::run(const Request& req, Result** out) {
   }
   const int passes = (bits_for(bin_total - 1) + kRadixBits - 1) / kRadixBits;
   const int64_t rtiles = radix_tiles(n);
-  const int64_t scan_len = (int64_t)K * n;
+  // work items: per system its atoms (brute-force rows), then
+  // (slot, stencil slice) pairs (cell-list home groups)
+  const int group = std::max(1, std::min(64, req.group > 0 ? req.group : config().group.load()));
+  const int parts = std::max(1, std::min(16, req.parts > 0 ? req.parts : config().parts.load()));
+  const int64_t n_items = n * (1 + parts);
+  std::vector<int64_t> item_base(S + 1);
+  for (int s = 0; s <= S; ++s) item_base[s] = req.starts[s] * (1 + parts);
+  const int64_t scan_len = (int64_t)K * n_items;
   const int64_t stiles = radix_tiles(scan_len);
 
   Params prm{};
@@ -221,6 +228,7 @@ int Engine::run(const Request& req, Result** out) {
                                  : (double*)scratch(kPos, sizeof(double) * 3 * n);
   SysDesc* d_sys = (SysDesc*)scratch(kSys, sizeof(SysDesc) * S);
   int64_t* d_start = (int64_t*)scratch(kStart, sizeof(int64_t) * (S + 1));
+  int64_t* d_items = (int64_t*)scratch(kItems, sizeof(int64_t) * (S + 1));
   double* d_rc2 = (double*)scratch(kRc2, sizeof(double) * K);
   float2* d_band = (float2*)scratch(kBand, sizeof(float2) * S * K);
   uint32_t* keysA = (uint32_t*)scratch(kKeysA, 4 * n);
@@ -244,8 +252,8 @@ int Engine::run(const Request& req, Result** out) {
   int64_t* bounds = (int64_t*)scratch(kBounds, 8 * (K + 1));
   const size_t pin_desc = sizeof(SysDesc) * S, pin_start = 8 * (S + 1), pin_rc2 = 8 * K;
   const size_t pin_bounds = 8 * (K + 1);
-  char* pin = (char*)pinned(pin_desc + pin_start + pin_rc2 + pin_bounds + 64);
-  if (!d_pos || !d_sys || !d_start || !d_rc2 || !d_band || !keysA || !valsA || !keysB || !valsB ||
+  char* pin = (char*)pinned(pin_desc + 2 * pin_start + pin_rc2 + pin_bounds + 64);
+  if (!d_pos || !d_sys || !d_start || !d_items || !d_rc2 || !d_band || !keysA || !valsA || !keysB || !valsB ||
       !zero || !cell_start || !spos64 || !spos32 || !sidx || !counts || !offsets || !bounds ||
       !pin) {
     set_error("out of device memory");
@@ -257,10 +265,12 @@ int Engine::run(const Request& req, Result** out) {
   uint32_t* tickets = rstatus + z_rstatus;
   char* h_desc = pin;
   char* h_start = h_desc + pin_desc;
-  char* h_rc2 = h_start + pin_start;
+  char* h_items = h_start + pin_start;
+  char* h_rc2 = h_items + pin_start;
   int64_t* h_bounds = (int64_t*)(h_rc2 + pin_rc2);
   std::memcpy(h_desc, desc.data(), pin_desc);
   std::memcpy(h_start, req.starts.data(), pin_start);
+  std::memcpy(h_items, item_base.data(), pin_start);
   std::memcpy(h_rc2, rc2.data(), pin_rc2);
 
   hipStream_t st = stream_;
@@ -268,6 +278,7 @@ int Engine::run(const Request& req, Result** out) {
     NL_CHECK(hipMemcpyAsync(d_pos, req.pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   NL_CHECK(hipMemcpyAsync(d_sys, h_desc, pin_desc, hipMemcpyHostToDevice, st));
   NL_CHECK(hipMemcpyAsync(d_start, h_start, pin_start, hipMemcpyHostToDevice, st));
+  NL_CHECK(hipMemcpyAsync(d_items, h_items, pin_start, hipMemcpyHostToDevice, st));
   NL_CHECK(hipMemcpyAsync(d_rc2, h_rc2, pin_rc2, hipMemcpyHostToDevice, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
 
@@ -290,6 +301,7 @@ int Engine::run(const Request& req, Result** out) {
   A.pos = d_pos;
   A.sys = d_sys;
   A.sys_start = d_start;
+  A.item_base = d_items;
   A.keys = kin;
   A.cell_start = cell_start;
   A.spos64 = spos64;
@@ -300,18 +312,20 @@ int Engine::run(const Request& req, Result** out) {
   A.counts = counts;
   A.offsets = offsets;
   A.bounds = bounds;
-  A.n = n;
+  A.n = n_items;
   A.n_sys = S;
   A.k = K;
   A.disjoint = req.disjoint;
   A.use_fp32 = prm.use_fp32;
   A.global_index = req.global_index;
+  A.group = group;
+  A.parts = parts;
   for (int q0 = 0; q0 < K; q0 += kGroup) {
     A.q_lo = q0;
     A.q_cnt = std::min(kGroup, K - q0);
     NL_CHECK(launch_pairs(false, A, st));
   }
-  NL_CHECK(launch_scan(counts, offsets, scan_len, n, K, bounds, sstatus, tickets + 4, st));
+  NL_CHECK(launch_scan(counts, offsets, scan_len, n_items, K, bounds, sstatus, tickets + 4, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[2], st));
 
   // Fill straight away when an estimate of the output size exists (same
@@ -359,7 +373,7 @@ int Engine::run(const Request& req, Result** out) {
       set_error("out of device memory");
       return 1;
     }
-    NL_CHECK(launch_sys_ranges(offsets, d_sys, S, K, n, d_ranges, st));
+    NL_CHECK(launch_sys_ranges(offsets, d_items, S, K, n_items, d_ranges, st));
     std::vector<int64_t> rr((size_t)2 * S * K);
     NL_CHECK(hipMemcpyAsync(rr.data(), d_ranges, 8 * rr.size(), hipMemcpyDeviceToHost, st));
     std::vector<SysDesc> dd(S);

@@ -20,6 +20,8 @@ struct Config {
   std::atomic<size_t> stack_threshold{1000};
   std::atomic<int> cell_div{2};  // GPU bin edge = rc_max / cell_div
   std::atomic<int> use_fp32{0};  // FP32 filter + FP64 band recheck
+  std::atomic<int> group{8};     // home atoms per cell-list work item
+  std::atomic<int> parts{2};     // half-stencil slices per home group
   std::atomic<int> autotune{1};
 };
 Config& config();
@@ -47,6 +49,8 @@ struct Request {
   bool want_dist = false;       // FP64 |r_ij| per edge
   int cell_div = 0;             // 0: engine default / tuned
   int use_fp32 = -1;            // -1: engine default
+  int group = 0;                // 0: engine default
+  int parts = 0;                // 0: engine default
 };
 
 class Engine;

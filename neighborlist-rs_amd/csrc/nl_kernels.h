@@ -13,6 +13,7 @@ struct PairArgs {
   const double* pos;         // raw positions (N,3), f64
   const SysDesc* sys;
   const int64_t* sys_start;  // n_sys + 1
+  const int64_t* item_base;  // n_sys + 1: work items of system s (atoms, then bins)
   const uint32_t* keys;      // sorted bin keys
   const int32_t* cell_start; // total_bins + 1
   const Pos64* spos64;       // sorted wrapped positions (FP64), w = local index
@@ -26,9 +27,11 @@ struct PairArgs {
   int64_t* edges;            // list q: sources at base_q + e, targets at base_q + M_q + e
   int32_t* shifts;           // list q: 3 per edge at 3 e
   double* dist;              // optional |r_ij| per edge (FP64), list q at e
-  int64_t n;
+  int64_t n;                 // work items
   int64_t capacity;          // edges the output buffers hold (fill pass)
-  int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index, pad;
+  int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index;
+  int32_t group;             // home atoms per cell-list work item (<= 64)
+  int32_t parts;             // slices of the half stencil per home group
 };
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
@@ -47,7 +50,7 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
 hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
                        int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st);
-hipError_t launch_sys_ranges(const int64_t* offsets, const SysDesc* sys, int n_sys, int k,
+hipError_t launch_sys_ranges(const int64_t* offsets, const int64_t* item_base, int n_sys, int k,
                              int64_t n, int64_t* ranges, hipStream_t st);
 
 inline int64_t radix_tiles(int64_t n) { return (n + kRadixTile - 1) / kRadixTile; }

@@ -438,46 +438,86 @@ struct WaveLds {
   float off32[3][64]; // H bs in FP32 (filter)
 };
 
-// index of the first list a squared distance falls in (cutoffs sorted
-// ascending): the number of cutoffs it is not strictly below
-__device__ inline int first_list(double d2, const double* __restrict__ rc2, int k) {
-  int q = 0;
-  for (int i = 0; i < k; ++i) q += !(d2 < rc2[i]);
-  return q;
+// The lists handled by one launch: q_lo .. q_lo+cnt-1 in ascending-cutoff
+// order.  Held in registers for the whole wave.  For cumulative lists a
+// pair belongs to list q iff d^2 < rc_q^2; with `disjoint` only to the
+// first list it reaches (search.rs:837-849), i.e. also not to q-1.
+// NQ: lists compiled into the kernel (1, 2, 3, 4 exactly; 8 = up to 8).
+template <int NQ>
+struct Cuts {
+  double r[NQ];         // rc^2 of the group's lists
+  double prev;          // rc^2 of list q_lo-1 (-inf when q_lo == 0)
+  float lo[NQ];         // FP32 certain-accept thresholds (per system)
+  float hi[NQ];         // FP32 certain-reject thresholds
+  float prev_lo, prev_hi;
+  int64_t base[NQ];     // first edge of each list (fill pass)
+  int64_t len[NQ];      // edges of each list
+  int cnt;
+  bool disjoint;
+  __device__ bool has(int g) const { return NQ < 8 || g < cnt; }
+};
+
+template <int NQ>
+__device__ inline uint32_t members64(double d2, const Cuts<NQ>& C) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int g = 0; g < NQ; ++g)
+    if (C.has(g) && d2 < C.r[g]) m |= 1u << g;
+  if (C.disjoint) m = (d2 < C.prev) ? 0u : (m & (0u - m));
+  return m;
 }
 
-// Emit or count one step of up to 64 candidates.  hit = valid & pair rule,
-// qs = first list reached.  `cur` holds the wave's running counts (count
-// pass) or write cursors (fill pass) for lists q_lo .. q_lo+q_cnt-1.
-template <bool FILL>
-__device__ inline void emit_step(const PairArgs& A, bool hit, int qs, int64_t gi, int64_t gj,
-                                 int sx, int sy, int sz, double dd, int64_t* cur) {
+// FP32 filter: the decision for each threshold is certain when e2 lies
+// below lo (inside) or above hi (outside); `unsure` when any needed
+// threshold is within its band, then the FP64 expression decides.
+template <int NQ>
+__device__ inline uint32_t members32(float e2, const Cuts<NQ>& C, bool* unsure) {
+  uint32_t m = 0;
+  bool u = false;
 #pragma unroll
-  for (int g = 0; g < kGroup; ++g) {
-    if (g >= A.q_cnt) break;
-    const int q = A.q_lo + g;
-    const bool mem = hit && (A.disjoint ? (qs == q) : (qs <= q));
-    const uint64_t m = __ballot(mem);
+  for (int g = 0; g < NQ; ++g) {
+    if (C.has(g)) {
+      if (e2 < C.lo[g]) m |= 1u << g;
+      u |= (e2 >= C.lo[g]) && (e2 <= C.hi[g]);
+    }
+  }
+  if (C.disjoint) {
+    u |= (e2 >= C.prev_lo) && (e2 <= C.prev_hi);
+    m = (e2 < C.prev_lo) ? 0u : (m & (0u - m));
+  }
+  *unsure = u;
+  return m;
+}
+
+// Emit or count one step of up to 64 candidates; `mem` holds the lane's
+// list bits.  `cur` holds the wave's running counts (count pass) or write
+// cursors (fill pass).
+template <bool FILL, int NQ>
+__device__ inline void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
+                                 int64_t gj, int sx, int sy, int sz, double dd, int64_t* cur) {
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    if (!C.has(g)) break;
+    const bool on = (mem >> g) & 1u;
+    const uint64_t m = __ballot(on);
     if (m == 0) continue;
-    if (FILL) {
-      if (mem) {
-        const int64_t e = cur[g] + lanes_below(m);
-        const int64_t base = A.bounds[q], len = A.bounds[q + 1] - base;
-        A.edges[base + e] = gi;
-        A.edges[base + len + e] = gj;
-        A.shifts[3 * e] = sx;
-        A.shifts[3 * e + 1] = sy;
-        A.shifts[3 * e + 2] = sz;
-        if (A.dist) A.dist[e] = dd;
-      }
+    if (FILL && on) {
+      const int64_t e = cur[g] + lanes_below(m);
+      A.edges[C.base[g] + e] = gi;
+      A.edges[C.base[g] + C.len[g] + e] = gj;
+      int32_t* sp = A.shifts + 3 * e;
+      sp[0] = sx;
+      sp[1] = sy;
+      sp[2] = sz;
+      if (A.dist) A.dist[e] = dd;
     }
     cur[g] += (int64_t)__popcll(m);
   }
 }
 
-template <bool FILL>
-__device__ void unit_bruteforce(const PairArgs& A, const SysDesc* d, int64_t u, int mode,
-                                int64_t* cur) {
+template <bool FILL, int NQ>
+__device__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
+                                int mode, int64_t* cur) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int n = d->n_atoms;
@@ -485,7 +525,13 @@ __device__ void unit_bruteforce(const PairArgs& A, const SysDesc* d, int64_t u, 
   const double* pi = A.pos + 3 * u;
   const double xi = pi[0], yi = pi[1], zi = pi[2];
   const int64_t gi = A.global_index ? u : (int64_t)i;
-  const int pb0 = d->pbc[0], pb1 = d->pbc[1], pb2 = d->pbc[2];
+  const bool pb0 = d->pbc[0], pb1 = d->pbc[1], pb2 = d->pbc[2];
+  double hv[9], hiv[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    hv[k] = d->h[k];
+    hiv[k] = d->hinv[k];
+  }
   for (int j0 = i + 1; j0 < n; j0 += 64) {
     const int j = j0 + lane;
     const bool valid = j < n;
@@ -502,7 +548,7 @@ __device__ void unit_bruteforce(const PairArgs& A, const SysDesc* d, int64_t u, 
       } else {
         // Cell::get_shift_and_displacement (cell.rs:74-94)
         double df[3];
-        matvec3(d->hinv, xj - xi, yj - yi, zj - zi, df);
+        matvec3(hiv, xj - xi, yj - yi, zj - zi, df);
         const double f0 = pb0 ? -round(df[0]) : 0.0;
         const double f1 = pb1 ? -round(df[1]) : 0.0;
         const double f2 = pb2 ? -round(df[2]) : 0.0;
@@ -510,183 +556,231 @@ __device__ void unit_bruteforce(const PairArgs& A, const SysDesc* d, int64_t u, 
         sy = as_i32(f1);
         sz = as_i32(f2);
         double v[3];
-        matvec3(d->h, f0, f1, f2, v);
+        matvec3(hv, f0, f1, f2, v);
         dx = (xj + v[0]) - xi;
         dy = (yj + v[1]) - yi;
         dz = (zj + v[2]) - zi;
       }
       d2 = norm2(dx, dy, dz);
     }
-    const int qs = first_list(d2, A.rc2, A.k);
-    const bool hit = valid && qs < A.k;
-    if (__ballot(hit) == 0) continue;
+    const uint32_t mem = valid ? members64(d2, C) : 0u;
+    if (__ballot(mem != 0) == 0) continue;
     const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
-    emit_step<FILL>(A, hit, qs, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
+    emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
   }
 }
 
-// Cell-list unit: one sorted slot p.  Half stencil: every unordered
-// (atom, atom, image) triple is examined once, from the slot whose stencil
-// offset to the other is lexicographically positive (or, inside the home
-// bin with no image shift, from the lower slot).  The reference examines it
-// from the lower original index with d = (w_j - w_i) + H bs; from the other
-// end the same pair gives (w_i - w_j) + H (-bs), which is the exact negation
-// of every rounded term (IEEE subtraction and products are sign-symmetric),
-// so d^2 and therefore every decision are bit-identical.  Edges are emitted
-// in the reference's orientation (lower original index first, S from it);
-// a self-image (same atom, bs != 0) is emitted for both S and -S, as the
-// reference emits both (search.rs:549, Q3).
-template <bool FILL>
-__device__ void unit_celllist(const PairArgs& A, const SysDesc* d, int s, int64_t p,
-                              WaveLds& L, int64_t* cur) {
+__device__ inline double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Cell-list work item: up to G home atoms of one bin (a "home group", one per
+// lane) and one slice of the half stencil.  Candidate atoms stream through
+// the lanes 64 at a time, each loaded once per item and tested against every
+// home atom in turn.
+//
+// Half stencil: every unordered (atom, atom, image) triple is examined once,
+// from the bin whose stencil offset to the other is lexicographically
+// positive (inside the home bin with no image: lower slot first).  The
+// reference examines a pair from the lower original index with
+// d = (w_j - w_i) + H bs (search.rs:540-543); from the other end the same
+// pair gives (w_i - w_j) + H (-bs), the exact negation of every rounded term
+// (IEEE subtraction and products are sign-symmetric), so d^2 and every
+// decision are bit-identical.  Edges are written in the reference's
+// orientation (lower original index first, S = s_j - s_i + bs from it,
+// search.rs:695-697); a self image (same atom, bs != 0) is written for both
+// S and -S, as the reference emits both (search.rs:549).
+template <bool FILL, int NQ>
+__device__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
+                          int part, WaveLds& L, int64_t* cur) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int nbx = d->nb[0], nby = d->nb[1], nbz = d->nb[2];
   const int nsx = d->ns[0], nsy = d->ns[1], nsz = d->ns[2];
   const int64_t bb = d->bin_base;
   const int lb = (int)((int64_t)A.keys[p] - bb);
+  const int bin_lo = A.cell_start[bb + lb];
+  // a group is up to G consecutive slots of one bin, aligned to the bin start
+  if (((int)p - bin_lo) % A.group != 0) return;
+  const int hs = (int)p, he = min(A.cell_start[bb + lb + 1], (int)p + A.group);
   const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
-  const Pos64 pi = A.spos64[p];
-  const int4 si = A.sidx[p];
-  const int oi = si.w;
-  const float4 pi32 = A.spos32[p];
   const double h0 = d->h[0], h1 = d->h[1], h2 = d->h[2], h3 = d->h[3], h4 = d->h[4],
                h5 = d->h[5], h6 = d->h[6], h7 = d->h[7], h8 = d->h[8];
-  const float* h32 = d->h32;
+  const float g0 = d->h32[0], g1 = d->h32[1], g2 = d->h32[2], g3 = d->h32[3], g4 = d->h32[4],
+              g5 = d->h32[5], g6 = d->h32[6], g7 = d->h32[7], g8 = d->h32[8];
   const int wx = 2 * nsx + 1, wy = 2 * nsy + 1, wz = 2 * nsz + 1;
   const int T = wx * wy * wz;
   const int tc = (nsx * wy + nsy) * wz + nsz;  // the (0,0,0) offset
-  const float2* band = A.band + (int64_t)s * A.k;
-  const float rej32 = band[A.k - 1].y;  // beyond the largest band: certain miss
-  for (int t0 = tc; t0 < T; t0 += 64) {
-    const int t = t0 + lane;
-    int len = 0, st = 0, bsx = 0, bsy = 0, bsz = 0;
-    double ox = 0, oy = 0, oz = 0;
-    if (t < T) {
-      const int dz = t % wz - nsz;
-      const int rest = t / wz;
-      const int dy = rest % wy - nsy;
-      const int dx = rest / wy - nsx;
-      int cx, cy, cz;
-      div_mod(bx + dx, nbx, &cx, &bsx);
-      div_mod(by + dy, nby, &cy, &bsy);
-      div_mod(bz + dz, nbz, &cz, &bsz);
-      const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
-      st = A.cell_start[bin];
-      const int en = A.cell_start[bin + 1];
-      if (t == tc) st = (int)p + 1;  // home bin, no image: slots after p only
-      len = en - st;
-      // (ox + oy) + oz of search.rs:498 = H (bsx, bsy, bsz) in gemv order
-      ox = (h0 * (double)bsx + h1 * (double)bsy) + h2 * (double)bsz;
-      oy = (h3 * (double)bsx + h4 * (double)bsy) + h5 * (double)bsz;
-      oz = (h6 * (double)bsx + h7 * (double)bsy) + h8 * (double)bsz;
+  // this item's slice of the half stencil [tc, T)
+  const int slice = (T - tc + A.parts - 1) / A.parts;
+  const int t_lo = tc + part * slice;
+  const int t_hi = min(T, t_lo + slice);
+  const bool fp32 = A.use_fp32;
+  const float rej32 = C.hi[C.cnt - 1];  // beyond the group's largest band: no list
+  for (int hb = hs; hb < he; hb += 64) {
+    const int hn = min(64, he - hb);
+    Pos64 hp = {0.0, 0.0, 0.0, 0.0};
+    int4 hsi = make_int4(0, 0, 0, 0);
+    float4 hp32 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < hn) {
+      hp = A.spos64[hb + lane];
+      hsi = A.sidx[hb + lane];
+      if (fp32) hp32 = A.spos32[hb + lane];
     }
-    const int incl = wave_incl_scan(len);
-    L.incl[lane] = incl;
-    L.start[lane] = st;
-    L.bs[0][lane] = bsx;
-    L.bs[1][lane] = bsy;
-    L.bs[2][lane] = bsz;
-    L.off[0][lane] = ox;
-    L.off[1][lane] = oy;
-    L.off[2][lane] = oz;
-    if (A.use_fp32) {
-      L.off32[0][lane] = (h32[0] * (float)bsx + h32[1] * (float)bsy) + h32[2] * (float)bsz;
-      L.off32[1][lane] = (h32[3] * (float)bsx + h32[4] * (float)bsy) + h32[5] * (float)bsz;
-      L.off32[2][lane] = (h32[6] * (float)bsx + h32[7] * (float)bsy) + h32[8] * (float)bsz;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int total = __shfl(incl, 63, 64);
-    int seg0 = 0;  // first segment that still holds candidates at c0
-    for (int c0 = 0; c0 < total; c0 += 64) {
-      while (L.incl[seg0] <= c0) ++seg0;  // uniform: every lane sees the same
-      const int idx = c0 + lane;
-      const bool valid = idx < total;
-      int sg = seg0;
-      if (valid)
-        while (L.incl[sg] <= idx) ++sg;
-      const int excl = sg > 0 ? L.incl[sg - 1] : 0;
-      const int j = L.start[sg] + (idx - excl);
-      const int sbx = L.bs[0][sg], sby = L.bs[1][sg], sbz = L.bs[2][sg];
-      int qs = A.k;
-      int oj = 0;
-      double d2 = INFINITY;
-      if (valid) {
-        if (A.use_fp32) {
-          const float4 c = A.spos32[j];
-          oj = __float_as_int(c.w);
-          const float dx = (c.x - pi32.x) + L.off32[0][sg];
-          const float dy = (c.y - pi32.y) + L.off32[1][sg];
-          const float dz = (c.z - pi32.z) + L.off32[2][sg];
-          const float e2 = (dx * dx + dy * dy) + dz * dz;
-          if (e2 <= rej32) {
-            // certain accept below lo_q, certain reject above hi_q, FP64
-            // recheck of the exact reference expression inside any band
-            int q32 = 0;
-            bool in_band = false;
-            for (int q = 0; q < A.k; ++q) {
-              const float2 bq = band[q];
-              q32 += e2 > bq.y;
-              in_band |= (e2 >= bq.x) && (e2 <= bq.y);
+    for (int t0 = t_lo; t0 < t_hi; t0 += 64) {
+      const int t = t0 + lane;
+      int len = 0, st = 0, bsx = 0, bsy = 0, bsz = 0;
+      double ox = 0, oy = 0, oz = 0;
+      if (t < t_hi) {
+        const int dz = t % wz - nsz;
+        const int rest = t / wz;
+        const int dy = rest % wy - nsy;
+        const int dx = rest / wy - nsx;
+        int cx, cy, cz;
+        div_mod(bx + dx, nbx, &cx, &bsx);
+        div_mod(by + dy, nby, &cy, &bsy);
+        div_mod(bz + dz, nbz, &cz, &bsz);
+        const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
+        st = A.cell_start[bin];
+        const int en = A.cell_start[bin + 1];
+        if (t == tc) st = hb;  // home bin, no image: slots from this group on
+        len = en - st;
+        // (ox + oy) + oz of search.rs:498 = H (bsx, bsy, bsz) in gemv order
+        ox = (h0 * (double)bsx + h1 * (double)bsy) + h2 * (double)bsz;
+        oy = (h3 * (double)bsx + h4 * (double)bsy) + h5 * (double)bsz;
+        oz = (h6 * (double)bsx + h7 * (double)bsy) + h8 * (double)bsz;
+      }
+      const int incl = wave_incl_scan(len);
+      L.incl[lane] = incl;
+      L.start[lane] = st;
+      L.bs[0][lane] = bsx;
+      L.bs[1][lane] = bsy;
+      L.bs[2][lane] = bsz;
+      L.off[0][lane] = ox;
+      L.off[1][lane] = oy;
+      L.off[2][lane] = oz;
+      if (fp32) {
+        L.off32[0][lane] = (g0 * (float)bsx + g1 * (float)bsy) + g2 * (float)bsz;
+        L.off32[1][lane] = (g3 * (float)bsx + g4 * (float)bsy) + g5 * (float)bsz;
+        L.off32[2][lane] = (g6 * (float)bsx + g7 * (float)bsy) + g8 * (float)bsz;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int total = __shfl(incl, 63, 64);
+      int seg0 = 0;  // first segment that still holds candidates at c0 (uniform)
+      for (int c0 = 0; c0 < total; c0 += 64) {
+        while (L.incl[seg0] <= c0) ++seg0;
+        const int idx = c0 + lane;
+        const bool valid = idx < total;
+        int sg = seg0;
+        if (valid)
+          while (L.incl[sg] <= idx) ++sg;
+        const int excl = sg > 0 ? L.incl[sg - 1] : 0;
+        const int j = L.start[sg] + (idx - excl);
+        const bool home_seg = (t0 == tc) && (sg == 0);
+        const double cox = L.off[0][sg], coy = L.off[1][sg], coz = L.off[2][sg];
+        const int sbx = L.bs[0][sg], sby = L.bs[1][sg], sbz = L.bs[2][sg];
+        Pos64 c = {0.0, 0.0, 0.0, 0.0};
+        float4 c32 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float fox = 0.f, foy = 0.f, foz = 0.f;
+        int4 cj = make_int4(0, 0, 0, 0);
+        int oj = 0;
+        if (valid) {
+          if (fp32) {
+            c32 = A.spos32[j];
+            oj = __float_as_int(c32.w);
+            fox = L.off32[0][sg];
+            foy = L.off32[1][sg];
+            foz = L.off32[2][sg];
+          } else {
+            c = A.spos64[j];
+            oj = (int)c.w;
+          }
+          if (FILL) cj = A.sidx[j];
+        }
+        for (int i = 0; i < hn; ++i) {
+          const int oi = __builtin_amdgcn_readlane(hsi.w, i);
+          const bool cand = valid && (!home_seg || j > hb + i);
+          uint32_t mem = 0;
+          double d2 = INFINITY;
+          if (fp32) {
+            const float xi = readlane_f(hp32.x, i), yi = readlane_f(hp32.y, i),
+                        zi = readlane_f(hp32.z, i);
+            const float dx = (c32.x - xi) + fox;
+            const float dy = (c32.y - yi) + foy;
+            const float dz = (c32.z - zi) + foz;
+            const float e2 = (dx * dx + dy * dy) + dz * dz;
+            bool unsure = false;
+            const bool near = cand && e2 <= rej32;
+            if (near) mem = members32(e2, C, &unsure);
+            const bool need64 = near && (unsure || A.dist != nullptr);
+            if (__ballot(need64)) {
+              const double xi64 = readlane_d(hp.x, i), yi64 = readlane_d(hp.y, i),
+                           zi64 = readlane_d(hp.z, i);
+              if (need64) {
+                // FP64 recheck: the reference expression, exactly
+                const Pos64 c64 = A.spos64[j];
+                d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy, (c64.z - zi64) + coz);
+                mem = members64(d2, C);
+              }
             }
-            if (in_band || A.dist) {
-              const Pos64 c64 = A.spos64[j];
-              const double ddx = (c64.x - pi.x) + L.off[0][sg];
-              const double ddy = (c64.y - pi.y) + L.off[1][sg];
-              const double ddz = (c64.z - pi.z) + L.off[2][sg];
-              d2 = norm2(ddx, ddy, ddz);
-              qs = first_list(d2, A.rc2, A.k);
-            } else {
-              qs = q32;
+          } else {
+            const double xi = readlane_d(hp.x, i), yi = readlane_d(hp.y, i),
+                         zi = readlane_d(hp.z, i);
+            // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
+            d2 = norm2((c.x - xi) + cox, (c.y - yi) + coy, (c.z - zi) + coz);
+            if (cand) mem = members64(d2, C);
+          }
+          if (__ballot(mem != 0) == 0) continue;
+          const bool self_img = (mem != 0) && (oj == oi);
+          int sx = 0, sy = 0, sz = 0;
+          int64_t gi = 0, gj = 0;
+          if (FILL) {
+            const int six = __builtin_amdgcn_readlane(hsi.x, i);
+            const int siy = __builtin_amdgcn_readlane(hsi.y, i);
+            const int siz = __builtin_amdgcn_readlane(hsi.z, i);
+            if (mem) {
+              sx = cj.x - six + sbx;
+              sy = cj.y - siy + sby;
+              sz = cj.z - siz + sbz;
+              int lo_i = oi, hi_i = oj;
+              if (oj < oi) {
+                lo_i = oj;
+                hi_i = oi;
+                sx = -sx;
+                sy = -sy;
+                sz = -sz;
+              }
+              gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
+              gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
             }
           }
-        } else {
-          const Pos64 c = A.spos64[j];
-          oj = (int)c.w;
-          // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-          const double dx = (c.x - pi.x) + L.off[0][sg];
-          const double dy = (c.y - pi.y) + L.off[1][sg];
-          const double dz = (c.z - pi.z) + L.off[2][sg];
-          d2 = norm2(dx, dy, dz);
-          qs = first_list(d2, A.rc2, A.k);
+          const double dd = A.dist ? sqrt(d2) : 0.0;
+          emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, dd, cur);
+          if (__ballot(self_img))
+            emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, dd, cur);
         }
       }
-      const bool hit = valid && qs < A.k;
-      if (__ballot(hit) == 0) continue;
-      const bool self_img = hit && (oj == oi);
-      int sx = 0, sy = 0, sz = 0;
-      int64_t gi = 0, gj = 0;
-      if (FILL && hit) {
-        // search.rs:695-697: S = s_j - s_i + bs, oriented from the lower index
-        const int4 sj = A.sidx[j];
-        sx = sj.x - si.x + sbx;
-        sy = sj.y - si.y + sby;
-        sz = sj.z - si.z + sbz;
-        int lo_i = oi, hi_i = oj;
-        if (oj < oi) {
-          lo_i = oj;
-          hi_i = oi;
-          sx = -sx;
-          sy = -sy;
-          sz = -sz;
-        }
-        gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
-        gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
-      }
-      const double dd = A.dist ? sqrt(d2) : 0.0;
-      emit_step<FILL>(A, hit, qs, gi, gj, sx, sy, sz, dd, cur);
-      if (__ballot(self_img))
-        emit_step<FILL>(A, self_img, qs, gi, gj, -sx, -sy, -sz, dd, cur);
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
 }  // namespace
 
-template <bool FILL>
+// Work items: system s owns items [item_base[s], item_base[s+1]): first its
+// n_s atoms (brute-force rows when the system takes the brute-force path),
+// then n_s * parts (sorted slot, stencil slice) items (home groups when it
+// takes the cell-list path; slots that do not start a group are empty).
+// Edges of one system are thus contiguous and systems come out in input
+// order, as batch.rs:135-144 concatenates them.
+template <bool FILL, int NQ>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   __shared__ WaveLds s_lds[kWaves];
   const int w = threadIdx.x >> 6;
@@ -695,20 +789,43 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   // fill pass launched before the host saw the counts: write nothing when
   // the lists do not fit; the host re-runs the fill with exact buffers
   if (FILL && A.bounds[A.k] > A.capacity) return;
-  const int s = uniform_i32(find_sys(A.sys_start, A.n_sys, u));
+  const int s = uniform_i32(find_sys(A.item_base, A.n_sys, u));
   const SysDesc* d = A.sys + s;
   const int mode = uniform_i32(d->mode);
-  int64_t cur[kGroup];
+  const int64_t local = u - A.item_base[s];
+  // this launch's lists, in registers
+  Cuts<NQ> C;
+  C.cnt = A.q_cnt;
+  C.disjoint = A.disjoint != 0;
+  C.prev = A.q_lo > 0 ? A.rc2[A.q_lo - 1] : -INFINITY;
+  const float2* band = A.band + (int64_t)s * A.k;
+  const float2 bp = A.q_lo > 0 ? band[A.q_lo - 1] : make_float2(-INFINITY, -INFINITY);
+  C.prev_lo = bp.x;
+  C.prev_hi = bp.y;
+  int64_t cur[NQ];
 #pragma unroll
-  for (int g = 0; g < kGroup; ++g)
-    cur[g] = (FILL && g < A.q_cnt) ? A.offsets[(int64_t)(A.q_lo + g) * A.n + u] : 0;
-  if (mode == kCellList)
-    unit_celllist<FILL>(A, d, s, u, s_lds[w], cur);
-  else
-    unit_bruteforce<FILL>(A, d, u, mode, cur);
+  for (int g = 0; g < NQ; ++g) {
+    const bool on = g < A.q_cnt;
+    const int q = A.q_lo + (on ? g : 0);
+    C.r[g] = on ? A.rc2[q] : -INFINITY;
+    const float2 b = band[q];
+    C.lo[g] = on ? b.x : -INFINITY;
+    C.hi[g] = on ? b.y : -INFINITY;
+    C.base[g] = FILL && on ? A.bounds[q] : 0;
+    C.len[g] = FILL && on ? A.bounds[q + 1] - A.bounds[q] : 0;
+    cur[g] = (FILL && on) ? A.offsets[(int64_t)q * A.n + u] : 0;
+  }
+  const int n_atoms = d->n_atoms;
+  if (local < n_atoms) {
+    if (mode != kCellList) unit_bruteforce<FILL, NQ>(A, C, d, d->atom_start + local, mode, cur);
+  } else if (mode == kCellList) {
+    const int64_t r = local - n_atoms;
+    const int64_t p = d->atom_start + r / A.parts;
+    unit_cell<FILL, NQ>(A, C, d, p, (int)(r % A.parts), s_lds[w], cur);
+  }
   if (!FILL && lane_id() == 0) {
 #pragma unroll
-    for (int g = 0; g < kGroup; ++g)
+    for (int g = 0; g < NQ; ++g)
       if (g < A.q_cnt) A.counts[(int64_t)(A.q_lo + g) * A.n + u] = (uint32_t)cur[g];
   }
 }
@@ -780,14 +897,14 @@ __global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
 }
 
 // Per-system edge ranges: ranges[(s*k + q)*2 + {0,1}].
-__global__ void k_sys_ranges(const int64_t* __restrict__ off, const SysDesc* __restrict__ sys,
-                             int n_sys, int k, int64_t n, int64_t* __restrict__ ranges) {
+__global__ void k_sys_ranges(const int64_t* __restrict__ off,
+                             const int64_t* __restrict__ item_base, int n_sys, int k, int64_t n,
+                             int64_t* __restrict__ ranges) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n_sys * k) return;
   const int s = (int)(t / k), q = (int)(t % k);
-  const int64_t a0 = sys[s].atom_start, a1 = a0 + sys[s].n_atoms;
-  ranges[2 * t] = off[(int64_t)q * n + a0];
-  ranges[2 * t + 1] = off[(int64_t)q * n + a1];
+  ranges[2 * t] = off[(int64_t)q * n + item_base[s]];
+  ranges[2 * t + 1] = off[(int64_t)q * n + item_base[s + 1]];
 }
 
 // ---------------------------------------------------------------------------
@@ -838,13 +955,24 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st) {
-  if (a.n <= 0) return hipSuccess;
+template <int NQ>
+static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
   const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
   if (fill)
-    hipLaunchKernelGGL(k_pairs<true>, g, b, 0, st, a);
+    hipLaunchKernelGGL((k_pairs<true, NQ>), g, b, 0, st, a);
   else
-    hipLaunchKernelGGL(k_pairs<false>, g, b, 0, st, a);
+    hipLaunchKernelGGL((k_pairs<false, NQ>), g, b, 0, st, a);
+}
+
+hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st) {
+  if (a.n <= 0) return hipSuccess;
+  switch (a.q_cnt) {
+    case 1: launch_pairs_nq<1>(fill, a, st); break;
+    case 2: launch_pairs_nq<2>(fill, a, st); break;
+    case 3: launch_pairs_nq<3>(fill, a, st); break;
+    case 4: launch_pairs_nq<4>(fill, a, st); break;
+    default: launch_pairs_nq<8>(fill, a, st); break;
+  }
   return hipGetLastError();
 }
 
@@ -856,12 +984,12 @@ hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t se
   return hipGetLastError();
 }
 
-hipError_t launch_sys_ranges(const int64_t* offsets, const SysDesc* sys, int n_sys, int k,
+hipError_t launch_sys_ranges(const int64_t* offsets, const int64_t* item_base, int n_sys, int k,
                              int64_t n, int64_t* ranges, hipStream_t st) {
   const int64_t t = (int64_t)n_sys * k;
   if (t <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sys_ranges, dim3(grid_for(t, 256)), dim3(256), 0, st, offsets, sys, n_sys,
-                     k, n, ranges);
+  hipLaunchKernelGGL(k_sys_ranges, dim3(grid_for(t, 256)), dim3(256), 0, st, offsets, item_base,
+                     n_sys, k, n, ranges);
   return hipGetLastError();
 }
 

@@ -62,6 +62,8 @@ _SIGS = {
     "nl_get_cell_div": (C.c_int, []),
     "nl_set_fp32_filter": (None, [C.c_int]),
     "nl_get_fp32_filter": (C.c_int, []),
+    "nl_set_work_split": (None, [C.c_int, C.c_int]),
+    "nl_get_work_split": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "nl_cell_inverse": (C.c_int, [dp, dp]),
     "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
     "nl_device_malloc": (C.c_void_p, [C.c_size_t]),

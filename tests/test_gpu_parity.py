@@ -196,3 +196,26 @@ def test_distances_fp64(nr, oracle):
     ref = np.sqrt((rij ** 2).sum(1))
     np.testing.assert_allclose(d, ref, rtol=0, atol=1e-12)
     assert np.all(d < 6.0)
+
+
+@pytest.mark.parametrize("group,parts", [(1, 1), (3, 2), (16, 4), (64, 3)])
+def test_work_split_does_not_change_results(nr, oracle, group, parts):
+    from neighborlist_rs import workloads as W
+    from neighborlist_rs._lib import lib
+    lib.nl_set_work_split(group, parts)
+    try:
+        p, H, _ = W.c2_ethanol()
+        for rc in (6.0, 14.0):
+            _print_case("split", group, parts, rc)
+            want = oracle.search_single(p, H, [1, 1, 1], rc)
+            got = _gpu_single(nr, p, H, [True] * 3, rc)
+            assert_same_edges(got, want, f"G={group} P={parts} rc={rc}")
+        p, sizes, cells, cut = W.c4_ragged_batch(n_systems=24, nmax=900, seed=12)
+        batch, rows = W.batch_arrays(sizes, cells)
+        ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+        want = oracle.search_batch_multi(p, batch, ocells, cut, False)
+        got = nr.build_neighborlists_batch_multi(p, batch, rows, list(cut))
+        for k in range(len(cut)):
+            assert_same_edges(as_tuple(got[k]), want[k], f"G={group} P={parts} list {k}")
+    finally:
+        lib.nl_set_work_split(8, 2)
