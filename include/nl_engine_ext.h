@@ -61,6 +61,38 @@ int nl_get_fp32_filter(void);
 void nl_set_work_split(int group, int parts);
 int nl_get_work_split(int* group, int* parts);
 
+/* Autotuning: on a new workload class (number of lists, multi, log2 of
+ * atoms, systems and max cutoff) the engine times a staged search over the
+ * four tunables above and keeps the fastest; results never change. */
+void nl_set_autotune(int on);
+int nl_get_autotune(void);
+void nl_clear_tuning(void);
+/* one line per tuned class into buf (truncated to len); returns the length */
+size_t nl_tune_report(char* buf, size_t len);
+
+/* ---- graph-captured steady state (batching form) -------------------------
+ * A plan fixes a batch shape (system sizes, cells, cutoffs) whose positions
+ * live at a fixed device address and change between runs (an MD loop, a
+ * serving slot).  nl_plan_create sizes the output (capacity <= 0: from one
+ * eager count, plus 1/8 headroom) and captures the whole sequence (uploads,
+ * plan, bin, radix sort, finalize, count, scan, fill, bounds download) in a
+ * hipGraph; nl_plan_run replays it with one launch.  When the lists outgrow
+ * the capacity the fill writes nothing, the plan grows its buffers,
+ * re-captures and replays once more (reported by nl_plan_stats). */
+typedef struct nl_plan nl_plan;
+int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                   const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                   const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
+                   int global_index, int64_t capacity, nl_plan** out);
+int nl_plan_run(nl_plan* p);
+int64_t nl_plan_num_edges(const nl_plan* p, size_t list);
+int nl_plan_device(const nl_plan* p, size_t list, const int64_t** edge_index,
+                   const int32_t** shifts);
+int nl_plan_copy(const nl_plan* p, size_t list, int64_t* edge_i, int64_t* edge_j,
+                 int32_t* shifts);
+int nl_plan_stats(const nl_plan* p, int64_t* replays, int64_t* recaptures, int64_t* capacity);
+void nl_plan_free(nl_plan* p);
+
 /* Host-side cell helpers with the reference's arithmetic (no GPU needed):
  * H^-1 by the cofactor formula of Cell::new (cell.rs:18-21; returns nonzero
  * with "Cell matrix is not invertible" when det == 0) and Cell::wrap

@@ -18,6 +18,12 @@ struct nl_result {
   Result* r;
 };
 
+struct nl_plan {
+  nl::Plan p;
+  nl::Engine* eng;
+  int64_t cap;
+};
+
 namespace {
 
 int wrap_result(Result* r, nl_result** out) {
@@ -371,6 +377,123 @@ int nl_get_work_split(int* group, int* parts) {
   if (parts) *parts = nl::config().parts.load();
   return 0;
 }
+
+void nl_set_autotune(int on) { nl::config().autotune.store(on != 0); }
+int nl_get_autotune(void) { return nl::config().autotune.load(); }
+void nl_clear_tuning(void) { nl::engine_for_current_device()->clear_tuning(); }
+size_t nl_tune_report(char* buf, size_t len) {
+  const std::string r = nl::engine_for_current_device()->tune_report();
+  if (buf && len) {
+    const size_t m = r.size() < len - 1 ? r.size() : len - 1;
+    std::memcpy(buf, r.data(), m);
+    buf[m] = 0;
+  }
+  return r.size();
+}
+
+int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                   const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                   const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
+                   int global_index, int64_t capacity, nl_plan** out) {
+  if (!out) {
+    nl::set_error("output pointer is NULL");
+    return 1;
+  }
+  *out = nullptr;
+  if (check_cutoffs(cutoffs, n_cutoffs, "All cutoffs must be positive and finite")) return 1;
+  if (n == 0 || n_cutoffs == 0) {
+    nl::set_error("a plan needs at least one atom and one cutoff");
+    return 1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    nl::set_error("no HIP device available (libnlmi355 runs on the GPU only)");
+    return 1;
+  }
+  Request req;
+  if (sizes_request(d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, req)) return 1;
+  req.pos_device = true;
+  req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
+  req.multi = multi != 0;
+  req.disjoint = disjoint != 0;
+  req.global_index = global_index != 0;
+  nl_plan* p = new nl_plan();
+  p->eng = nl::engine_for_current_device();
+  int rc;
+  {
+    std::lock_guard<std::mutex> g(p->eng->mutex());
+    rc = p->p.create(p->eng, req, capacity);
+  }
+  if (rc != 0) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
+int nl_plan_run(nl_plan* p) {
+  if (!p) {
+    nl::set_error("invalid plan");
+    return 1;
+  }
+  return p->p.run();
+}
+
+int64_t nl_plan_num_edges(const nl_plan* p, size_t list) {
+  if (!p || list >= (size_t)p->p.pipe().K) return -1;
+  return p->p.len((int)list);
+}
+
+int nl_plan_device(const nl_plan* p, size_t list, const int64_t** edge_index,
+                   const int32_t** shifts) {
+  if (!p || list >= (size_t)p->p.pipe().K) {
+    nl::set_error("invalid plan or list index");
+    return 1;
+  }
+  const int64_t b = p->p.base((int)list);
+  if (edge_index) *edge_index = (const int64_t*)p->p.edges() + 2 * b;
+  if (shifts) *shifts = (const int32_t*)p->p.shifts() + 3 * b;
+  return 0;
+}
+
+int nl_plan_copy(const nl_plan* p, size_t list, int64_t* edge_i, int64_t* edge_j,
+                 int32_t* shifts) {
+  if (!p || list >= (size_t)p->p.pipe().K) {
+    nl::set_error("invalid plan or list index");
+    return 1;
+  }
+  const int64_t m = p->p.len((int)list), b = p->p.base((int)list);
+  if (m == 0) return 0;
+  hipStream_t st = p->p.stream();
+  const int64_t* e = (const int64_t*)p->p.edges();
+  hipError_t err = hipSuccess;
+  if (edge_i) err = hipMemcpyAsync(edge_i, e + 2 * b, 8 * m, hipMemcpyDeviceToHost, st);
+  if (edge_j && err == hipSuccess)
+    err = hipMemcpyAsync(edge_j, e + 2 * b + m, 8 * m, hipMemcpyDeviceToHost, st);
+  if (shifts && err == hipSuccess)
+    err = hipMemcpyAsync(shifts, (const int32_t*)p->p.shifts() + 3 * b, 12 * m,
+                         hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (err != hipSuccess) {
+    nl::set_error(std::string("HIP error in nl_plan_copy: ") + hipGetErrorString(err));
+    return 1;
+  }
+  return 0;
+}
+
+int nl_plan_stats(const nl_plan* p, int64_t* replays, int64_t* recaptures, int64_t* capacity) {
+  if (!p) {
+    nl::set_error("invalid plan");
+    return 1;
+  }
+  if (replays) *replays = p->p.replays;
+  if (recaptures) *recaptures = p->p.recaptures;
+  if (capacity) *capacity = p->p.capacity();
+  return 0;
+}
+
+void nl_plan_free(nl_plan* p) { delete p; }
 
 int nl_cell_inverse(const double* h, double* hinv) {
   if (!nl::inverse3(h, hinv)) {

@@ -1,12 +1,15 @@
-// nl_engine.cpp -- host engine: request planning, scratch buffers, the
-// launch sequence and result objects.  One Engine per device; all calls on
-// a device are serialised by the engine mutex.
+// nl_engine.cpp -- host engine: request preparation (Pipeline), scratch and
+// output pools, the eager launch path, the autotuner and graph-captured
+// plans.  One Engine per device; all calls on a device are serialised by the
+// engine mutex.
 #include "nl_engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <sstream>
 
 namespace nl {
 
@@ -26,11 +29,252 @@ Result::~Result() {
   if (dist) eng->dev_release(dist, dist_bytes);
 }
 
-// scratch slots
-enum Slot {
-  kPos, kSys, kStart, kItems, kRc2, kBand, kKeysA, kValsA, kKeysB, kValsB, kZero, kCellStart, kSpos64,
-  kSpos32, kSidx, kCounts, kOffsets, kBounds, kRanges, kSlots
-};
+static int bits_for(int64_t v) {
+  int b = 1;
+  while (b < 32 && (int64_t(1) << b) <= v) ++b;
+  return b;
+}
+
+#define NL_TRY(call)                 \
+  do {                               \
+    hipError_t e_ = (call);          \
+    if (e_ != hipSuccess) return e_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Pipeline
+
+int Pipeline::prepare(const Request& req, const Tuning& t) {
+  n = req.n;
+  S = (int)req.sys.size();
+  K = (int)req.cutoffs.size();
+  tune = t;
+  if (K <= 0 || K > kMaxCutoffs) {
+    set_error("number of cutoffs must be between 1 and 64");
+    return 1;
+  }
+  pos_device = req.pos_device;
+  host_pos = req.pos;
+  global_index = req.global_index;
+  disjoint = req.disjoint;
+  // cutoffs: stable ascending order (Rust sort_by is stable, search.rs:354-356),
+  // squares in f64 (search.rs:357-360)
+  order.resize(K);
+  for (int q = 0; q < K; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return req.cutoffs[a] < req.cutoffs[b]; });
+  rc2.resize(K);
+  for (int q = 0; q < K; ++q) rc2[q] = req.cutoffs[order[q]] * req.cutoffs[order[q]];
+  double rc_max = NAN;  // NaN-ignoring fold (single.rs:106)
+  for (double c : req.cutoffs) rc_max = std::isnan(rc_max) ? c : std::fmax(rc_max, c);
+
+  // per-system descriptors (static part), bin reservation
+  desc.assign(S, SysDesc{});
+  bin_total = 0;
+  for (int s = 0; s < S; ++s) {
+    SysDesc& d = desc[s];
+    std::memset(&d, 0, sizeof d);
+    d.atom_start = req.starts[s];
+    d.n_atoms = (int32_t)(req.starts[s + 1] - req.starts[s]);
+    d.has_cell = req.sys[s].has_cell;
+    for (int k = 0; k < 3; ++k) d.pbc[k] = req.sys[s].has_cell ? (req.sys[s].pbc[k] != 0) : 0;
+    if (d.has_cell) {
+      std::memcpy(d.h, req.sys[s].h, sizeof d.h);
+      double inv[9];
+      if (!inverse3(d.h, inv)) {
+        set_error("Cell matrix is not invertible");  // cell.rs:6
+        return 1;
+      }
+    }
+    const int64_t cap = std::min<int64_t>(int64_t(1) << 22, 2 * (int64_t)d.n_atoms + 64);
+    d.bin_cap = (int32_t)cap;
+    d.bin_base = bin_total;
+    bin_total += cap;
+  }
+  if (bin_total >= (int64_t(1) << 31) || n >= (int64_t(1) << 28)) {
+    set_error("request too large for one launch (more than 2^28 atoms or 2^31 bins)");
+    return 1;
+  }
+  passes = (bits_for(bin_total - 1) + kRadixBits - 1) / kRadixBits;
+  rtiles = radix_tiles(n);
+  // work items: per system its atoms (brute-force rows), then
+  // (slot, stencil slice) pairs (cell-list home groups)
+  tune.group = std::max(1, std::min(64, tune.group));
+  tune.parts = std::max(1, std::min(16, tune.parts));
+  tune.cell_div = std::max(1, std::min(4, tune.cell_div));
+  n_items = n * (1 + tune.parts);
+  starts = req.starts;
+  item_base.resize(S + 1);
+  for (int s = 0; s <= S; ++s) item_base[s] = starts[s] * (1 + tune.parts);
+  scan_len = (int64_t)K * n_items;
+  stiles = radix_tiles(scan_len);
+  // one zero-initialised block: scan look-back status (u64, first), radix
+  // histograms, radix look-back status, tile tickets
+  z_words = 2 * stiles + 256 * passes + 256 * rtiles * passes + 8;
+
+  prm = Params{};
+  prm.n_atoms = n;
+  prm.n_sys = S;
+  prm.k = K;
+  prm.disjoint = req.disjoint;
+  prm.multi = req.multi;
+  const uint64_t thr = config().bf_threshold.load();
+  prm.bf_threshold_lo = (int32_t)(uint32_t)(thr & 0xffffffffu);
+  prm.bf_threshold_hi = (int32_t)(uint32_t)(thr >> 32);
+  prm.cell_div = tune.cell_div;
+  prm.use_fp32 = tune.use_fp32;
+  prm.global_index = req.global_index;
+  prm.rc_max = rc_max;
+  return 0;
+}
+
+std::vector<size_t> Pipeline::buffer_sizes() const {
+  return {
+      pos_device ? 0 : sizeof(double) * 3 * (size_t)n,              // pos
+      sizeof(SysDesc) * S,                                          // sys
+      8 * (size_t)(S + 1),                                          // start
+      8 * (size_t)(S + 1),                                          // items
+      8 * (size_t)K,                                                // rc2
+      sizeof(float2) * (size_t)S * K,                               // band
+      4 * (size_t)n, 4 * (size_t)n, 4 * (size_t)n, 4 * (size_t)n,  // keys/vals A/B
+      4 * (size_t)z_words,                                          // zero
+      4 * (size_t)(bin_total + 1),                                  // cell_start
+      sizeof(Pos64) * (size_t)n,                                    // spos64
+      sizeof(float4) * (size_t)n,                                   // spos32
+      sizeof(int4) * (size_t)n,                                     // sidx
+      4 * (size_t)scan_len,                                         // counts
+      8 * (size_t)(scan_len + 1),                                   // offsets
+      8 * (size_t)(K + 1),                                          // bounds
+  };
+}
+
+void Pipeline::assign(Buffers& b, const std::vector<void*>& p) {
+  b.pos = (double*)p[0];
+  b.sys = (SysDesc*)p[1];
+  b.start = (int64_t*)p[2];
+  b.items = (int64_t*)p[3];
+  b.rc2 = (double*)p[4];
+  b.band = (float2*)p[5];
+  b.keysA = (uint32_t*)p[6];
+  b.valsA = (uint32_t*)p[7];
+  b.keysB = (uint32_t*)p[8];
+  b.valsB = (uint32_t*)p[9];
+  b.zero = (uint32_t*)p[10];
+  b.cell_start = (int32_t*)p[11];
+  b.spos64 = (Pos64*)p[12];
+  b.spos32 = (float4*)p[13];
+  b.sidx = (int4*)p[14];
+  b.counts = (uint32_t*)p[15];
+  b.offsets = (int64_t*)p[16];
+  b.bounds = (int64_t*)p[17];
+}
+
+size_t Pipeline::pinned_bytes() const { return off_bounds() + 8 * (size_t)(K + 1) + 64; }
+
+void Pipeline::stage_host(char* pin) const {
+  std::memcpy(pin, desc.data(), sizeof(SysDesc) * S);
+  std::memcpy(pin + off_start(), starts.data(), 8 * (size_t)(S + 1));
+  std::memcpy(pin + off_items(), item_base.data(), 8 * (size_t)(S + 1));
+  std::memcpy(pin + off_rc2(), rc2.data(), 8 * (size_t)K);
+}
+
+const int64_t* Pipeline::host_bounds(const char* pin) const {
+  return (const int64_t*)(pin + off_bounds());
+}
+
+int Pipeline::rank_of(int list) const {
+  return (int)(std::find(order.begin(), order.end(), list) - order.begin());
+}
+
+hipError_t Pipeline::enqueue_upload(const Buffers& b, const char* pin, hipStream_t st) const {
+  if (!pos_device)
+    NL_TRY(hipMemcpyAsync(b.pos, host_pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  NL_TRY(hipMemcpyAsync(b.sys, pin, sizeof(SysDesc) * S, hipMemcpyHostToDevice, st));
+  NL_TRY(hipMemcpyAsync(b.start, pin + off_start(), 8 * (S + 1), hipMemcpyHostToDevice, st));
+  NL_TRY(hipMemcpyAsync(b.items, pin + off_items(), 8 * (S + 1), hipMemcpyHostToDevice, st));
+  NL_TRY(hipMemcpyAsync(b.rc2, pin + off_rc2(), 8 * K, hipMemcpyHostToDevice, st));
+  return hipSuccess;
+}
+
+static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
+  PairArgs A{};
+  A.pos = b.pos;
+  A.sys = b.sys;
+  A.sys_start = b.start;
+  A.item_base = b.items;
+  // the sorted keys end in A after an even number of radix passes
+  A.keys = (P.passes % 2 == 0) ? b.keysA : b.keysB;
+  A.cell_start = b.cell_start;
+  A.spos64 = b.spos64;
+  A.spos32 = b.spos32;
+  A.sidx = b.sidx;
+  A.rc2 = b.rc2;
+  A.band = b.band;
+  A.counts = b.counts;
+  A.offsets = b.offsets;
+  A.bounds = b.bounds;
+  A.n = P.n_items;
+  A.n_sys = P.S;
+  A.k = P.K;
+  A.disjoint = P.disjoint;
+  A.use_fp32 = P.tune.use_fp32;
+  A.global_index = P.global_index;
+  A.group = P.tune.group;
+  A.parts = P.tune.parts;
+  return A;
+}
+
+hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const {
+  uint64_t* sstatus = (uint64_t*)b.zero;
+  uint32_t* hist = b.zero + 2 * stiles;
+  uint32_t* rstatus = hist + 256 * passes;
+  uint32_t* tickets = rstatus + 256 * rtiles * passes;
+  NL_TRY(launch_fill_u32(b.zero, z_words, 0, st));
+  NL_TRY(launch_plan(b.pos, b.sys, b.band, b.rc2, prm, st));
+  NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, hist, passes, st));
+  uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
+  for (int ps = 0; ps < passes; ++ps) {
+    NL_TRY(launch_radix_pass(kin, vin, kout, vout, n, kRadixBits * ps, hist + 256 * ps,
+                             rstatus + (int64_t)256 * rtiles * ps, tickets + ps, st));
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  NL_TRY(launch_finalize(kin, vin, b.pos, b.sys, b.start, S, n, bin_total, b.cell_start,
+                         b.spos64, b.spos32, b.sidx, st));
+  PairArgs A = pair_args(*this, b);
+  for (int q0 = 0; q0 < K; q0 += kGroup) {
+    A.q_lo = q0;
+    A.q_cnt = std::min(kGroup, K - q0);
+    NL_TRY(launch_pairs(false, A, st));
+  }
+  NL_TRY(launch_scan(b.counts, b.offsets, scan_len, n_items, K, b.bounds, sstatus, tickets + 4,
+                     st));
+  return hipSuccess;
+}
+
+hipError_t Pipeline::enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
+                                  int64_t capacity, hipStream_t st) const {
+  PairArgs A = pair_args(*this, b);
+  A.edges = (int64_t*)edges;
+  A.shifts = (int32_t*)shifts;
+  A.dist = (double*)dist;
+  A.capacity = capacity;
+  for (int q0 = 0; q0 < K; q0 += kGroup) {
+    A.q_lo = q0;
+    A.q_cnt = std::min(kGroup, K - q0);
+    NL_TRY(launch_pairs(true, A, st));
+  }
+  return hipSuccess;
+}
+
+hipError_t Pipeline::enqueue_bounds(const Buffers& b, char* pin, hipStream_t st) const {
+  return hipMemcpyAsync(pin + off_bounds(), b.bounds, 8 * (K + 1), hipMemcpyDeviceToHost, st);
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+
+enum { kRangesSlot = kNumBuffers, kSlots };
 
 Engine::Engine(int device) : device_(device), scratch_(kSlots) {
   (void)hipSetDevice(device_);
@@ -78,27 +322,6 @@ void* Engine::pinned(size_t bytes) {
   return pinned_;
 }
 
-int Engine::alloc_outputs(Result* res, int64_t edges, bool dist) {
-  if (res->edges) dev_release(res->edges, res->edges_bytes);
-  if (res->shifts) dev_release(res->shifts, res->shifts_bytes);
-  if (res->dist) dev_release(res->dist, res->dist_bytes);
-  res->edges = res->shifts = res->dist = nullptr;
-  const int64_t m = std::max<int64_t>(edges, 1);
-  res->edges_bytes = sizeof(int64_t) * 2 * m;
-  res->shifts_bytes = sizeof(int32_t) * 3 * m;
-  res->edges = dev_alloc(res->edges_bytes);
-  res->shifts = dev_alloc(res->shifts_bytes);
-  if (dist) {
-    res->dist_bytes = sizeof(double) * m;
-    res->dist = dev_alloc(res->dist_bytes);
-  }
-  if (!res->edges || !res->shifts || (dist && !res->dist)) {
-    set_error("out of device memory for the edge lists");
-    return 1;
-  }
-  return 0;
-}
-
 void* Engine::dev_alloc(size_t bytes) {
   bytes = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
   {
@@ -128,222 +351,91 @@ void Engine::dev_release(void* p, size_t bytes) {
   pool_.emplace(bytes, p);
 }
 
-static int bits_for(int64_t v) {
-  int b = 1;
-  while (b < 32 && (int64_t(1) << b) <= v) ++b;
-  return b;
+int Engine::alloc_outputs(Result* res, int64_t edges, bool dist) {
+  if (res->edges) dev_release(res->edges, res->edges_bytes);
+  if (res->shifts) dev_release(res->shifts, res->shifts_bytes);
+  if (res->dist) dev_release(res->dist, res->dist_bytes);
+  res->edges = res->shifts = res->dist = nullptr;
+  const int64_t m = std::max<int64_t>(edges, 1);
+  res->edges_bytes = sizeof(int64_t) * 2 * m;
+  res->shifts_bytes = sizeof(int32_t) * 3 * m;
+  res->edges = dev_alloc(res->edges_bytes);
+  res->shifts = dev_alloc(res->shifts_bytes);
+  if (dist) {
+    res->dist_bytes = sizeof(double) * m;
+    res->dist = dev_alloc(res->dist_bytes);
+  }
+  if (!res->edges || !res->shifts || (dist && !res->dist)) {
+    set_error("out of device memory for the edge lists");
+    return 1;
+  }
+  return 0;
 }
 
-#define NL_CHECK(call)                           \
-  do {                                           \
-    hipError_t e_ = (call);                      \
+#define NL_CHECK(call)                            \
+  do {                                            \
+    hipError_t e_ = (call);                       \
     if (e_ != hipSuccess) return fail(e_, #call); \
   } while (0)
 
 int Engine::run(const Request& req, Result** out) {
   *out = nullptr;
-  const int64_t n = req.n;
-  const int S = (int)req.sys.size();
+  return run_with(req, tuning_for(req), out);
+}
+
+int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
+  *out = nullptr;
   const int K = (int)req.cutoffs.size();
-  if (K <= 0 || K > kMaxCutoffs) {
-    set_error("number of cutoffs must be between 1 and 64");
-    return 1;
-  }
   std::unique_ptr<Result> res(new Result());
   res->eng = this;
   res->k = K;
   res->len.assign(K, 0);
   res->base.assign(K, 0);
-  if (n == 0 || S == 0) {
+  res->tune = t;
+  if (req.n == 0 || req.sys.empty()) {
     *out = res.release();
     return 0;
   }
   (void)hipSetDevice(device_);
-
-  // cutoffs: stable ascending order (Rust sort_by is stable), squares in f64
-  std::vector<int> order(K);
-  for (int q = 0; q < K; ++q) order[q] = q;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return req.cutoffs[a] < req.cutoffs[b]; });
-  std::vector<double> rc2(K);
-  for (int q = 0; q < K; ++q) rc2[q] = req.cutoffs[order[q]] * req.cutoffs[order[q]];
-  double rc_max = NAN;  // NaN-ignoring fold (single.rs:106)
-  for (double c : req.cutoffs) rc_max = std::isnan(rc_max) ? c : std::fmax(rc_max, c);
-
-  // per-system descriptors (static part), bin reservation
-  std::vector<SysDesc> desc(S);
-  int64_t bin_total = 0;
-  for (int s = 0; s < S; ++s) {
-    SysDesc& d = desc[s];
-    std::memset(&d, 0, sizeof d);
-    d.atom_start = req.starts[s];
-    d.n_atoms = (int32_t)(req.starts[s + 1] - req.starts[s]);
-    d.has_cell = req.sys[s].has_cell;
-    for (int k = 0; k < 3; ++k) d.pbc[k] = req.sys[s].has_cell ? (req.sys[s].pbc[k] != 0) : 0;
-    if (d.has_cell) {
-      std::memcpy(d.h, req.sys[s].h, sizeof d.h);
-      double inv[9];
-      if (!inverse3(d.h, inv)) {
-        set_error("Cell matrix is not invertible");  // cell.rs:6
-        return 1;
-      }
+  Pipeline P;
+  if (P.prepare(req, t)) return 1;
+  const int S = P.S;
+  std::vector<size_t> sz = P.buffer_sizes();
+  std::vector<void*> ptr(kNumBuffers);
+  for (int i = 0; i < kNumBuffers; ++i) {
+    if (i == 0 && P.pos_device) {
+      ptr[0] = const_cast<double*>(req.pos);
+      continue;
     }
-    const int64_t cap = std::min<int64_t>(int64_t(1) << 22, 2 * (int64_t)d.n_atoms + 64);
-    d.bin_cap = (int32_t)cap;
-    d.bin_base = bin_total;
-    bin_total += cap;
+    ptr[i] = scratch(i, sz[i]);
+    if (!ptr[i]) {
+      set_error("out of device memory");
+      return 1;
+    }
   }
-  if (bin_total >= (int64_t(1) << 31) || n >= (int64_t(1) << 31)) {
-    set_error("request too large for one launch (more than 2^31 atoms or bins)");
+  Buffers B;
+  Pipeline::assign(B, ptr);
+  char* pin = (char*)pinned(P.pinned_bytes());
+  if (!pin) {
+    set_error("out of pinned host memory");
     return 1;
   }
-  const int passes = (bits_for(bin_total - 1) + kRadixBits - 1) / kRadixBits;
-  const int64_t rtiles = radix_tiles(n);
-  // work items: per system its atoms (brute-force rows), then
-  // (slot, stencil slice) pairs (cell-list home groups)
-  const int group = std::max(1, std::min(64, req.group > 0 ? req.group : config().group.load()));
-  const int parts = std::max(1, std::min(16, req.parts > 0 ? req.parts : config().parts.load()));
-  const int64_t n_items = n * (1 + parts);
-  std::vector<int64_t> item_base(S + 1);
-  for (int s = 0; s <= S; ++s) item_base[s] = req.starts[s] * (1 + parts);
-  const int64_t scan_len = (int64_t)K * n_items;
-  const int64_t stiles = radix_tiles(scan_len);
-
-  Params prm{};
-  prm.n_atoms = n;
-  prm.n_sys = S;
-  prm.k = K;
-  prm.disjoint = req.disjoint;
-  prm.multi = req.multi;
-  const uint64_t thr = config().bf_threshold.load();
-  prm.bf_threshold_lo = (int32_t)(uint32_t)(thr & 0xffffffffu);
-  prm.bf_threshold_hi = (int32_t)(uint32_t)(thr >> 32);
-  prm.cell_div = req.cell_div > 0 ? req.cell_div : config().cell_div.load();
-  prm.use_fp32 = req.use_fp32 >= 0 ? req.use_fp32 : config().use_fp32.load();
-  prm.global_index = req.global_index;
-  prm.rc_max = rc_max;
-
-  // scratch
-  double* d_pos = req.pos_device ? const_cast<double*>(req.pos)
-                                 : (double*)scratch(kPos, sizeof(double) * 3 * n);
-  SysDesc* d_sys = (SysDesc*)scratch(kSys, sizeof(SysDesc) * S);
-  int64_t* d_start = (int64_t*)scratch(kStart, sizeof(int64_t) * (S + 1));
-  int64_t* d_items = (int64_t*)scratch(kItems, sizeof(int64_t) * (S + 1));
-  double* d_rc2 = (double*)scratch(kRc2, sizeof(double) * K);
-  float2* d_band = (float2*)scratch(kBand, sizeof(float2) * S * K);
-  uint32_t* keysA = (uint32_t*)scratch(kKeysA, 4 * n);
-  uint32_t* valsA = (uint32_t*)scratch(kValsA, 4 * n);
-  uint32_t* keysB = (uint32_t*)scratch(kKeysB, 4 * n);
-  uint32_t* valsB = (uint32_t*)scratch(kValsB, 4 * n);
-  // one zero-initialised block: radix histograms, radix look-back status,
-  // tile tickets, scan look-back status (64-bit aligned first)
-  const int64_t z_sstatus = 2 * stiles;             // u32 words
-  const int64_t z_hist = 256 * passes;
-  const int64_t z_rstatus = 256 * rtiles * passes;
-  const int64_t z_tickets = 8;
-  const int64_t z_words = z_sstatus + z_hist + z_rstatus + z_tickets;
-  uint32_t* zero = (uint32_t*)scratch(kZero, 4 * z_words);
-  int32_t* cell_start = (int32_t*)scratch(kCellStart, 4 * (bin_total + 1));
-  Pos64* spos64 = (Pos64*)scratch(kSpos64, sizeof(Pos64) * n);
-  float4* spos32 = (float4*)scratch(kSpos32, sizeof(float4) * n);
-  int4* sidx = (int4*)scratch(kSidx, sizeof(int4) * n);
-  uint32_t* counts = (uint32_t*)scratch(kCounts, 4 * scan_len);
-  int64_t* offsets = (int64_t*)scratch(kOffsets, 8 * (scan_len + 1));
-  int64_t* bounds = (int64_t*)scratch(kBounds, 8 * (K + 1));
-  const size_t pin_desc = sizeof(SysDesc) * S, pin_start = 8 * (S + 1), pin_rc2 = 8 * K;
-  const size_t pin_bounds = 8 * (K + 1);
-  char* pin = (char*)pinned(pin_desc + 2 * pin_start + pin_rc2 + pin_bounds + 64);
-  if (!d_pos || !d_sys || !d_start || !d_items || !d_rc2 || !d_band || !keysA || !valsA || !keysB || !valsB ||
-      !zero || !cell_start || !spos64 || !spos32 || !sidx || !counts || !offsets || !bounds ||
-      !pin) {
-    set_error("out of device memory");
-    return 1;
-  }
-  uint64_t* sstatus = (uint64_t*)zero;
-  uint32_t* hist = zero + z_sstatus;
-  uint32_t* rstatus = hist + z_hist;
-  uint32_t* tickets = rstatus + z_rstatus;
-  char* h_desc = pin;
-  char* h_start = h_desc + pin_desc;
-  char* h_items = h_start + pin_start;
-  char* h_rc2 = h_items + pin_start;
-  int64_t* h_bounds = (int64_t*)(h_rc2 + pin_rc2);
-  std::memcpy(h_desc, desc.data(), pin_desc);
-  std::memcpy(h_start, req.starts.data(), pin_start);
-  std::memcpy(h_items, item_base.data(), pin_start);
-  std::memcpy(h_rc2, rc2.data(), pin_rc2);
+  P.stage_host(pin);
 
   hipStream_t st = stream_;
-  if (!req.pos_device)
-    NL_CHECK(hipMemcpyAsync(d_pos, req.pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_sys, h_desc, pin_desc, hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_start, h_start, pin_start, hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_items, h_items, pin_start, hipMemcpyHostToDevice, st));
-  NL_CHECK(hipMemcpyAsync(d_rc2, h_rc2, pin_rc2, hipMemcpyHostToDevice, st));
+  NL_CHECK(P.enqueue_upload(B, pin, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
-
-  // build: plan, bin, radix sort, finalize
-  NL_CHECK(launch_fill_u32(zero, z_words, 0, st));
-  NL_CHECK(launch_plan(d_pos, d_sys, d_band, d_rc2, prm, st));
-  NL_CHECK(launch_bin(d_pos, d_sys, d_start, S, n, keysA, valsA, hist, passes, st));
-  uint32_t *kin = keysA, *vin = valsA, *kout = keysB, *vout = valsB;
-  for (int ps = 0; ps < passes; ++ps) {
-    NL_CHECK(launch_radix_pass(kin, vin, kout, vout, n, kRadixBits * ps, hist + 256 * ps,
-                               rstatus + (int64_t)256 * rtiles * ps, tickets + ps, st));
-    std::swap(kin, kout);
-    std::swap(vin, vout);
-  }
-  NL_CHECK(launch_finalize(kin, vin, d_pos, d_sys, d_start, S, n, bin_total, cell_start, spos64,
-                           spos32, sidx, st));
-  if (timing_) NL_CHECK(hipEventRecord(ev_[1], st));
-
-  PairArgs A{};
-  A.pos = d_pos;
-  A.sys = d_sys;
-  A.sys_start = d_start;
-  A.item_base = d_items;
-  A.keys = kin;
-  A.cell_start = cell_start;
-  A.spos64 = spos64;
-  A.spos32 = spos32;
-  A.sidx = sidx;
-  A.rc2 = d_rc2;
-  A.band = d_band;
-  A.counts = counts;
-  A.offsets = offsets;
-  A.bounds = bounds;
-  A.n = n_items;
-  A.n_sys = S;
-  A.k = K;
-  A.disjoint = req.disjoint;
-  A.use_fp32 = prm.use_fp32;
-  A.global_index = req.global_index;
-  A.group = group;
-  A.parts = parts;
-  for (int q0 = 0; q0 < K; q0 += kGroup) {
-    A.q_lo = q0;
-    A.q_cnt = std::min(kGroup, K - q0);
-    NL_CHECK(launch_pairs(false, A, st));
-  }
-  NL_CHECK(launch_scan(counts, offsets, scan_len, n_items, K, bounds, sstatus, tickets + 4, st));
+  NL_CHECK(P.enqueue_build_count(B, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[2], st));
 
   // Fill straight away when an estimate of the output size exists (same
   // request shape as last time); the kernel writes nothing if it is short.
-  const int64_t sig[4] = {n, S, K, (int64_t)req.multi * 2 + req.disjoint};
+  const int64_t sig[4] = {P.n, S, K, (int64_t)req.multi * 2 + req.disjoint};
   const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
-  int64_t cap = have_est ? last_total_ + last_total_ / 8 + 4096 : -1;
+  const int64_t cap = have_est ? last_total_ + last_total_ / 8 + 4096 : -1;
   auto launch_fill = [&](int64_t capacity) -> int {
-    A.edges = (int64_t*)res->edges;
-    A.shifts = (int32_t*)res->shifts;
-    A.dist = (double*)res->dist;
-    A.capacity = capacity;
     if (timing_) NL_CHECK(hipEventRecord(ev_[3], st));
-    for (int q0 = 0; q0 < K; q0 += kGroup) {
-      A.q_lo = q0;
-      A.q_cnt = std::min(kGroup, K - q0);
-      NL_CHECK(launch_pairs(true, A, st));
-    }
+    NL_CHECK(P.enqueue_fill(B, res->edges, res->shifts, res->dist, capacity, st));
     if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
     return 0;
   };
@@ -351,9 +443,9 @@ int Engine::run(const Request& req, Result** out) {
     if (alloc_outputs(res.get(), cap, req.want_dist)) return 1;
     if (launch_fill(cap)) return 1;
   }
-  NL_CHECK(hipMemcpyAsync(h_bounds, bounds, pin_bounds, hipMemcpyDeviceToHost, st));
+  NL_CHECK(P.enqueue_bounds(B, pin, st));
   NL_CHECK(hipStreamSynchronize(st));
-  std::vector<int64_t> hb(h_bounds, h_bounds + K + 1);
+  const int64_t* hb = P.host_bounds(pin);
   const int64_t total = hb[K];
   res->total = total;
   if (!have_est || total > cap) {
@@ -363,28 +455,28 @@ int Engine::run(const Request& req, Result** out) {
   for (int q = 0; q < 4; ++q) last_sig_[q] = sig[q];
   last_total_ = total;
   for (int q = 0; q < K; ++q) {
-    const int r = (int)(std::find(order.begin(), order.end(), q) - order.begin());
+    const int r = P.rank_of(q);
     res->base[q] = hb[r];
     res->len[q] = hb[r + 1] - hb[r];
   }
   if (req.want_ranges) {
-    int64_t* d_ranges = (int64_t*)scratch(kRanges, 8 * 2 * (int64_t)S * K);
+    int64_t* d_ranges = (int64_t*)scratch(kRangesSlot, 8 * 2 * (int64_t)S * K);
     if (!d_ranges) {
       set_error("out of device memory");
       return 1;
     }
-    NL_CHECK(launch_sys_ranges(offsets, d_items, S, K, n_items, d_ranges, st));
+    NL_CHECK(launch_sys_ranges(B.offsets, B.items, S, K, P.n_items, d_ranges, st));
     std::vector<int64_t> rr((size_t)2 * S * K);
     NL_CHECK(hipMemcpyAsync(rr.data(), d_ranges, 8 * rr.size(), hipMemcpyDeviceToHost, st));
     std::vector<SysDesc> dd(S);
-    NL_CHECK(hipMemcpyAsync(dd.data(), d_sys, sizeof(SysDesc) * S, hipMemcpyDeviceToHost, st));
+    NL_CHECK(hipMemcpyAsync(dd.data(), B.sys, sizeof(SysDesc) * S, hipMemcpyDeviceToHost, st));
     NL_CHECK(hipStreamSynchronize(st));
     res->ranges.assign((size_t)2 * S * K, 0);
     res->modes.resize(S);
     for (int s = 0; s < S; ++s) {
       res->modes[s] = dd[s].mode;
       for (int q = 0; q < K; ++q) {
-        const int r = (int)(std::find(order.begin(), order.end(), q) - order.begin());
+        const int r = P.rank_of(q);
         res->ranges[2 * ((size_t)s * K + q)] = rr[2 * ((size_t)s * K + r)];
         res->ranges[2 * ((size_t)s * K + q) + 1] = rr[2 * ((size_t)s * K + r) + 1];
       }
@@ -392,24 +484,123 @@ int Engine::run(const Request& req, Result** out) {
   }
   NL_CHECK(hipStreamSynchronize(st));
   if (timing_) {
-    float t = 0;
-    (void)hipEventElapsedTime(&t, ev_[0], ev_[1]);
-    times_.build = t;
-    (void)hipEventElapsedTime(&t, ev_[1], ev_[2]);
-    times_.count = t;
+    float tm = 0;
+    (void)hipEventElapsedTime(&tm, ev_[0], ev_[2]);
+    times_.build = 0;
+    times_.count = tm;  // build + count pass + scan
     times_.fill = 0;
     if (total > 0) {
-      (void)hipEventElapsedTime(&t, ev_[3], ev_[4]);
-      times_.fill = t;
+      (void)hipEventElapsedTime(&tm, ev_[3], ev_[4]);
+      times_.fill = tm;
     }
-    (void)hipEventElapsedTime(&t, ev_[0], total > 0 ? ev_[4] : ev_[2]);
-    times_.total = t;
-    times_.units = n;
+    (void)hipEventElapsedTime(&tm, ev_[0], total > 0 ? ev_[4] : ev_[2]);
+    times_.total = tm;
+    times_.units = P.n_items;
     times_.edges = total;
-    times_.bins = bin_total;
+    times_.bins = P.bin_total;
   }
   *out = res.release();
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: per workload class (lists, multi, log2 atoms, log2 systems,
+// log2 max cutoff) a staged search over bin size, home-group size, stencil
+// slices and the FP32 filter, timing full launches.  None of the knobs can
+// change a result, only the time it takes.
+
+static int lg2(double x) { return x <= 1.0 ? 0 : (int)std::floor(std::log2(x)); }
+
+Tuning Engine::tuning_for(const Request& req) {
+  Tuning t;
+  Config& c = config();
+  t.cell_div = req.cell_div > 0 ? req.cell_div : c.cell_div.load();
+  t.use_fp32 = req.use_fp32 >= 0 ? req.use_fp32 : c.use_fp32.load();
+  t.group = req.group > 0 ? req.group : c.group.load();
+  t.parts = req.parts > 0 ? req.parts : c.parts.load();
+  const bool explicit_req =
+      req.cell_div > 0 || req.use_fp32 >= 0 || req.group > 0 || req.parts > 0;
+  if (!c.autotune.load() || explicit_req || req.n < 2048) return t;
+  double rc_max = 0;
+  for (double x : req.cutoffs) rc_max = std::max(rc_max, x);
+  const auto key = std::make_tuple((int)req.cutoffs.size(), (int)req.multi, lg2((double)req.n),
+                                   lg2((double)req.sys.size()), lg2(rc_max));
+  {
+    std::lock_guard<std::mutex> g(tune_mu_);
+    auto it = tuned_.find(key);
+    if (it != tuned_.end()) return it->second;
+  }
+  Request probe = req;
+  probe.want_ranges = false;
+  probe.want_dist = false;
+  auto time_one = [&](const Tuning& x) -> double {
+    double best = 1e30;
+    for (int rep = 0; rep < 2; ++rep) {
+      Result* r = nullptr;
+      auto t0 = std::chrono::steady_clock::now();
+      if (run_with(probe, x, &r) != 0) return 1e30;
+      auto t1 = std::chrono::steady_clock::now();
+      delete r;
+      best = std::min(best, std::chrono::duration<double>(t1 - t0).count());
+    }
+    return best;
+  };
+  Tuning best = t;
+  double tb = time_one(best);
+  auto consider = [&](Tuning x) {
+    const double tx = time_one(x);
+    if (tx < tb) {
+      tb = tx;
+      best = x;
+    }
+  };
+  for (int dv : {1, 2, 3, 4}) {
+    if (dv == t.cell_div) continue;
+    Tuning x = best;
+    x.cell_div = dv;
+    consider(x);
+  }
+  const Tuning after_div = best;
+  for (int g : {1, 2, 8, 16}) {
+    if (g == after_div.group) continue;
+    Tuning x = after_div;
+    x.group = g;
+    consider(x);
+  }
+  const Tuning after_g = best;
+  for (int p : {1, 2, 4}) {
+    if (p == after_g.parts) continue;
+    Tuning x = after_g;
+    x.parts = p;
+    consider(x);
+  }
+  {
+    Tuning x = best;
+    x.use_fp32 = 1 - best.use_fp32;
+    consider(x);
+  }
+  std::lock_guard<std::mutex> g(tune_mu_);
+  tuned_[key] = best;
+  return best;
+}
+
+std::string Engine::tune_report() {
+  std::lock_guard<std::mutex> g(tune_mu_);
+  std::ostringstream o;
+  for (auto& kv : tuned_) {
+    const auto& k = kv.first;
+    const Tuning& t = kv.second;
+    o << "k=" << std::get<0>(k) << " multi=" << std::get<1>(k) << " log2n=" << std::get<2>(k)
+      << " log2sys=" << std::get<3>(k) << " log2rc=" << std::get<4>(k)
+      << " -> cell_div=" << t.cell_div << " group=" << t.group << " parts=" << t.parts
+      << " fp32=" << t.use_fp32 << "\n";
+  }
+  return o.str();
+}
+
+void Engine::clear_tuning() {
+  std::lock_guard<std::mutex> g(tune_mu_);
+  tuned_.clear();
 }
 
 Engine* engine_for_current_device() {
@@ -423,6 +614,133 @@ Engine* engine_for_current_device() {
   Engine* e = new Engine(dev);
   engines[dev] = e;
   return e;
+}
+
+// ---------------------------------------------------------------------------
+// Plan: graph-captured steady state
+
+Plan::~Plan() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  for (void* p : owned_)
+    if (p) (void)hipFree(p);
+  if (edges_) (void)hipFree(edges_);
+  if (shifts_) (void)hipFree(shifts_);
+  if (pinned_) (void)hipHostFree(pinned_);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+int Plan::alloc_out(int64_t cap) {
+  if (edges_) (void)hipFree(edges_);
+  if (shifts_) (void)hipFree(shifts_);
+  edges_ = shifts_ = nullptr;
+  cap_ = std::max<int64_t>(cap, 1);
+  if (hipMalloc(&edges_, 16 * (size_t)cap_) != hipSuccess ||
+      hipMalloc(&shifts_, 12 * (size_t)cap_) != hipSuccess) {
+    set_error("out of device memory for the plan's edge lists");
+    return 1;
+  }
+  return 0;
+}
+
+int Plan::capture() {
+  if (exec_) {
+    (void)hipGraphExecDestroy(exec_);
+    exec_ = nullptr;
+  }
+  if (graph_) {
+    (void)hipGraphDestroy(graph_);
+    graph_ = nullptr;
+  }
+  hipError_t e = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return eng_->fail(e, "hipStreamBeginCapture");
+  e = pipe_.enqueue_upload(buf_, pinned_, st_);
+  if (e == hipSuccess) e = pipe_.enqueue_build_count(buf_, st_);
+  if (e == hipSuccess) e = pipe_.enqueue_fill(buf_, edges_, shifts_, nullptr, cap_, st_);
+  if (e == hipSuccess) e = pipe_.enqueue_bounds(buf_, pinned_, st_);
+  hipError_t e2 = hipStreamEndCapture(st_, &graph_);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+  if (e != hipSuccess) return eng_->fail(e, "graph capture");
+  ++recaptures;
+  return 0;
+}
+
+int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
+  eng_ = eng;
+  (void)hipSetDevice(eng->device());
+  if (!req.pos_device) {
+    set_error("a plan needs device-resident positions");
+    return 1;
+  }
+  const Tuning t = eng->tuning_for(req);
+  if (pipe_.prepare(req, t)) return 1;
+  std::vector<size_t> sz = pipe_.buffer_sizes();
+  std::vector<void*> ptr(kNumBuffers, nullptr);
+  ptr[0] = const_cast<double*>(req.pos);
+  for (int i = 1; i < kNumBuffers; ++i) {
+    if (hipMalloc(&ptr[i], std::max<size_t>(sz[i], 16)) != hipSuccess) {
+      set_error("out of device memory for the plan");
+      return 1;
+    }
+    owned_.push_back(ptr[i]);
+  }
+  Pipeline::assign(buf_, ptr);
+  if (hipHostMalloc((void**)&pinned_, pipe_.pinned_bytes(), hipHostMallocDefault) !=
+      hipSuccess) {
+    pinned_ = nullptr;
+    set_error("out of pinned host memory");
+    return 1;
+  }
+  pipe_.stage_host(pinned_);
+  if (hipStreamCreateWithFlags(&st_, hipStreamNonBlocking) != hipSuccess) {
+    set_error("stream creation failed");
+    return 1;
+  }
+  if (capacity <= 0) {
+    // size the lists from one eager count
+    hipError_t e = pipe_.enqueue_upload(buf_, pinned_, st_);
+    if (e == hipSuccess) e = pipe_.enqueue_build_count(buf_, st_);
+    if (e == hipSuccess) e = pipe_.enqueue_bounds(buf_, pinned_, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return eng_->fail(e, "plan sizing");
+    const int64_t tot = pipe_.host_bounds(pinned_)[pipe_.K];
+    capacity = tot + tot / 8 + 4096;
+  }
+  if (alloc_out(capacity)) return 1;
+  return capture();
+}
+
+int Plan::run() {
+  (void)hipSetDevice(eng_->device());
+  hipError_t e = hipGraphLaunch(exec_, st_);
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  if (e != hipSuccess) return eng_->fail(e, "graph launch");
+  ++replays;
+  const int64_t* hb = pipe_.host_bounds(pinned_);
+  total_ = hb[pipe_.K];
+  if (total_ > cap_) {
+    // overflow: the fill wrote nothing; grow, re-capture, replay once more
+    if (alloc_out(total_ + total_ / 4 + 4096)) return 1;
+    if (capture()) return 1;
+    e = hipGraphLaunch(exec_, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return eng_->fail(e, "graph launch");
+    ++replays;
+    total_ = hb[pipe_.K];
+  }
+  bounds_.assign(hb, hb + pipe_.K + 1);
+  return 0;
+}
+
+int64_t Plan::len(int list) const {
+  const int r = pipe_.rank_of(list);
+  return bounds_.empty() ? 0 : bounds_[r + 1] - bounds_[r];
+}
+
+int64_t Plan::base(int list) const {
+  const int r = pipe_.rank_of(list);
+  return bounds_.empty() ? 0 : bounds_[r];
 }
 
 }  // namespace nl

@@ -1,11 +1,13 @@
-// nl_engine.h -- host side of the engine: per-device state, the launch
-// sequence for one (possibly ragged, multi-cutoff) request, result objects.
+// nl_engine.h -- host side of the engine: request preparation, the launch
+// sequence (Pipeline), per-device state and scratch (Engine), graph-captured
+// steady-state plans (Plan), result objects and the autotuner.
 #pragma once
 #include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "nl_common.h"
@@ -13,16 +15,16 @@
 
 namespace nl {
 
-// Process-wide thresholds (config.rs:4-39 semantics).
+// Process-wide thresholds (config.rs:4-39 semantics) and engine defaults.
 struct Config {
   std::atomic<size_t> bf_threshold{1000};
   std::atomic<size_t> par_threshold{300};
   std::atomic<size_t> stack_threshold{1000};
-  std::atomic<int> cell_div{2};  // GPU bin edge = rc_max / cell_div
+  std::atomic<int> cell_div{3};  // GPU bin edge = rc_max / cell_div
   std::atomic<int> use_fp32{0};  // FP32 filter + FP64 band recheck
-  std::atomic<int> group{8};     // home atoms per cell-list work item
-  std::atomic<int> parts{2};     // half-stencil slices per home group
-  std::atomic<int> autotune{1};
+  std::atomic<int> group{4};     // home atoms per cell-list work item
+  std::atomic<int> parts{1};     // half-stencil slices per home group
+  std::atomic<int> autotune{1};  // pick the four above per workload class
 };
 Config& config();
 
@@ -47,10 +49,74 @@ struct Request {
   bool global_index = true;
   bool want_ranges = false;     // per-system edge ranges
   bool want_dist = false;       // FP64 |r_ij| per edge
-  int cell_div = 0;             // 0: engine default / tuned
-  int use_fp32 = -1;            // -1: engine default
-  int group = 0;                // 0: engine default
-  int parts = 0;                // 0: engine default
+  int cell_div = 0;             // 0: tuned / engine default
+  int use_fp32 = -1;            // -1: tuned / engine default
+  int group = 0;                // 0: tuned / engine default
+  int parts = 0;                // 0: tuned / engine default
+};
+
+struct Tuning {
+  int cell_div = 3, use_fp32 = 0, group = 4, parts = 1;
+};
+
+// Device buffers of one launch sequence (owned by an Engine or a Plan).
+struct Buffers {
+  double* pos = nullptr;
+  SysDesc* sys = nullptr;
+  int64_t* start = nullptr;
+  int64_t* items = nullptr;
+  double* rc2 = nullptr;
+  float2* band = nullptr;
+  uint32_t *keysA = nullptr, *valsA = nullptr, *keysB = nullptr, *valsB = nullptr;
+  uint32_t* zero = nullptr;
+  int32_t* cell_start = nullptr;
+  Pos64* spos64 = nullptr;
+  float4* spos32 = nullptr;
+  int4* sidx = nullptr;
+  uint32_t* counts = nullptr;
+  int64_t* offsets = nullptr;
+  int64_t* bounds = nullptr;
+};
+constexpr int kNumBuffers = 18;
+
+// Host-prepared launch sequence for one request shape.
+class Pipeline {
+ public:
+  // Host work only: validation of cells, sorted cutoffs, descriptors, sizes.
+  int prepare(const Request& req, const Tuning& t);
+
+  // sizes of each device buffer (bytes), in Buffers order (pos first)
+  std::vector<size_t> buffer_sizes() const;
+  static void assign(Buffers& b, const std::vector<void*>& ptrs);
+  size_t pinned_bytes() const;
+  void stage_host(char* pinned) const;  // descriptors etc. into pinned memory
+
+  // Stages, all asynchronous on `st`.
+  hipError_t enqueue_upload(const Buffers& b, const char* pinned, hipStream_t st) const;
+  hipError_t enqueue_build_count(const Buffers& b, hipStream_t st) const;
+  hipError_t enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
+                          int64_t capacity, hipStream_t st) const;
+  hipError_t enqueue_bounds(const Buffers& b, char* pinned, hipStream_t st) const;
+  const int64_t* host_bounds(const char* pinned) const;
+  int rank_of(int list) const;  // input list -> sorted rank
+
+  int64_t n = 0, n_items = 0, bin_total = 0, scan_len = 0, rtiles = 0, stiles = 0,
+          z_words = 0;
+  int S = 0, K = 0, passes = 0;
+  Tuning tune;
+  Params prm{};
+  bool pos_device = false, global_index = true, disjoint = false;
+  const double* host_pos = nullptr;
+  std::vector<int> order;  // sorted rank -> input list
+  std::vector<double> rc2; // sorted
+  std::vector<SysDesc> desc;
+  std::vector<int64_t> starts, item_base;
+
+ private:
+  size_t off_start() const { return sizeof(SysDesc) * S; }
+  size_t off_items() const { return off_start() + 8 * (size_t)(S + 1); }
+  size_t off_rc2() const { return off_items() + 8 * (size_t)(S + 1); }
+  size_t off_bounds() const { return off_rc2() + 8 * (size_t)K; }
 };
 
 class Engine;
@@ -62,14 +128,15 @@ struct Result {
   std::vector<int64_t> len;    // per list, input order
   std::vector<int64_t> base;   // first global edge of each list, input order
   int64_t total = 0;
-  void* edges = nullptr;       // int64 [2 * total]
+  void* edges = nullptr;       // int64 [2 * capacity]
   size_t edges_bytes = 0;
-  void* shifts = nullptr;      // int32 [3 * total]
+  void* shifts = nullptr;      // int32 [3 * capacity]
   size_t shifts_bytes = 0;
-  void* dist = nullptr;        // double [total] (optional)
+  void* dist = nullptr;        // double [capacity] (optional)
   size_t dist_bytes = 0;
   std::vector<int64_t> ranges; // [s][list(input order)][2] global edge ids (optional)
   std::vector<int32_t> modes;  // per system (optional, with ranges)
+  Tuning tune;
   ~Result();
 };
 
@@ -87,28 +154,33 @@ class Engine {
 
   // Runs the whole pipeline; on success *out owns device buffers.
   int run(const Request& req, Result** out);
+  // Picks the tunables for the request (autotuning on a new workload class).
+  Tuning tuning_for(const Request& req);
 
   void* dev_alloc(size_t bytes);
   void dev_release(void* p, size_t bytes);
   StageTimes last_times() const { return times_; }
   void set_timing(bool on) { timing_ = on; }
   std::mutex& mutex() { return mu_; }
-
- private:
-  struct Scratch {
-    void* p = nullptr;
-    size_t cap = 0;
-  };
-  void* scratch(int slot, size_t bytes);
-  void* pinned(size_t bytes);
   int fail(hipError_t e, const char* where);
   int alloc_outputs(Result* res, int64_t edges, bool dist);
+  std::string tune_report();
+  void clear_tuning();
+
+ private:
+  int run_with(const Request& req, const Tuning& t, Result** out);
+  void* scratch(int slot, size_t bytes);
+  void* pinned(size_t bytes);
 
   int device_;
   hipStream_t stream_ = nullptr;
   std::mutex mu_;
   std::mutex pool_mu_;
   std::multimap<size_t, void*> pool_;
+  struct Scratch {
+    void* p = nullptr;
+    size_t cap = 0;
+  };
   std::vector<Scratch> scratch_;
   hipEvent_t ev_[6] = {};
   bool timing_ = false;
@@ -119,8 +191,45 @@ class Engine {
   // launched without waiting for the counts
   int64_t last_sig_[4] = {-1, -1, -1, -1};
   int64_t last_total_ = -1;
+  std::mutex tune_mu_;
+  std::map<std::tuple<int, int, int, int, int>, Tuning> tuned_;
 };
 
 Engine* engine_for_current_device();
+
+// Graph-captured steady state: a fixed batch shape (systems, cells, cutoffs)
+// whose positions live at a fixed device address and change between runs
+// (a molecular-dynamics loop, a serving slot).  One hipGraph launch per run.
+class Plan {
+ public:
+  ~Plan();
+  int create(Engine* eng, const Request& req, int64_t capacity);
+  int run();  // replay; grows outputs (re-capture) when too small
+  int64_t total() const { return total_; }
+  int64_t capacity() const { return cap_; }
+  int64_t len(int list) const;
+  int64_t base(int list) const;
+  const Pipeline& pipe() const { return pipe_; }
+  void* edges() const { return edges_; }
+  void* shifts() const { return shifts_; }
+  hipStream_t stream() const { return st_; }
+  int replays = 0, recaptures = 0;
+
+ private:
+  int capture();
+  int alloc_out(int64_t cap);
+  Engine* eng_ = nullptr;
+  Pipeline pipe_;
+  Buffers buf_;
+  std::vector<void*> owned_;
+  char* pinned_ = nullptr;
+  hipStream_t st_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  void* edges_ = nullptr;
+  void* shifts_ = nullptr;
+  int64_t cap_ = 0, total_ = 0;
+  std::vector<int64_t> bounds_;
+};
 
 }  // namespace nl

@@ -64,6 +64,20 @@ _SIGS = {
     "nl_get_fp32_filter": (C.c_int, []),
     "nl_set_work_split": (None, [C.c_int, C.c_int]),
     "nl_get_work_split": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "nl_set_autotune": (None, [C.c_int]),
+    "nl_get_autotune": (C.c_int, []),
+    "nl_clear_tuning": (None, []),
+    "nl_tune_report": (C.c_size_t, [C.c_char_p, C.c_size_t]),
+    "nl_plan_create": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp,
+                                 C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int64,
+                                 C.POINTER(C.c_void_p)]),
+    "nl_plan_run": (C.c_int, [C.c_void_p]),
+    "nl_plan_num_edges": (C.c_int64, [C.c_void_p, C.c_size_t]),
+    "nl_plan_device": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p)]),
+    "nl_plan_copy": (C.c_int, [C.c_void_p, C.c_size_t, i64p, i64p, i32p]),
+    "nl_plan_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p]),
+    "nl_plan_free": (None, [C.c_void_p]),
     "nl_cell_inverse": (C.c_int, [dp, dp]),
     "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
     "nl_device_malloc": (C.c_void_p, [C.c_size_t]),

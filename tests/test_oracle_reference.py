@@ -320,7 +320,11 @@ def _multi_image_bruteforce(pos, H, rc):
     i < j or (i == j, S != 0)."""
     inv = np.linalg.inv(H)
     w = np.min(1.0 / np.linalg.norm(inv, axis=1))
-    m = int(np.ceil(rc / w)) + 1
+    # positions need not lie in the cell: widen the image range by their
+    # fractional spread
+    frac = pos @ inv.T
+    spread = float((frac.max(0) - frac.min(0)).max()) if len(pos) else 0.0
+    m = int(np.ceil(rc / w + spread)) + 1
     g = np.arange(-m, m + 1)
     S = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
     off = S @ H.T
