@@ -87,6 +87,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tuning", default="auto",
+                    help="'auto' (autotuner) or cell_div,group,parts,fp32 pinned")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="run only the timed batch steps (for rocprofv3)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run a device-to-device copy of this size (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
@@ -97,6 +103,22 @@ def main():
     from neighborlist_rs import workloads as W
     lib = L.lib
     lib.nl_set_timing(1)
+    if args.tuning != "auto":
+        dv, g, pp, f32 = (int(x) for x in args.tuning.split(","))
+        lib.nl_set_autotune(0)
+        lib.nl_set_cell_div(dv)
+        lib.nl_set_work_split(g, pp)
+        lib.nl_set_fp32_filter(f32)
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        a = lib.nl_device_malloc(nb)
+        b = lib.nl_device_malloc(nb)
+        for _ in range(3):
+            lib.nl_memcpy_d2d(C.c_void_p(b), C.c_void_p(a), nb)
+        lib.nl_device_synchronize()
+        lib.nl_device_free(C.c_void_p(a))
+        lib.nl_device_free(C.c_void_p(b))
+        log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB device-to-device")
 
     # ---- headline: ragged batch, three cutoffs, one pass ------------------
     pos, sizes, cells, cut = W.c4_ragged_batch()
@@ -160,6 +182,14 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                 "kernel": "k_pairs<fill>", "kernel_ms": round(fill_ms, 4),
                 "alg_bytes_per_launch": int(alg_bytes)}
+
+    if args.only_headline:
+        print(json.dumps({"metric": "edges/s", "value": round(value, 1), "unit": "edges/s",
+                          "ms_per_step": round(ms_per_step, 4), "roofline": roofline,
+                          "tuning": tuned, "eager_batched_ms_per_step": round(eager_ms, 4)}),
+              flush=True)
+        dpos.free()
+        return
 
     # ---- plain path: one call per system (same inputs, device-resident) --
     starts = np.concatenate([[0], np.cumsum(sizes)])

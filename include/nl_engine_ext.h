@@ -106,6 +106,7 @@ void* nl_device_malloc(size_t bytes);
 int nl_device_free(void* p);
 int nl_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int nl_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int nl_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int nl_device_synchronize(void);
 
 /* device selection for subsequent calls on this thread */

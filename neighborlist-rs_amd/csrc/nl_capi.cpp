@@ -537,6 +537,9 @@ int nl_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 int nl_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
+int nl_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+}
 int nl_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
 int nl_set_device(int device) {

@@ -458,7 +458,7 @@ struct Cuts {
 };
 
 template <int NQ>
-__device__ inline uint32_t members64(double d2, const Cuts<NQ>& C) {
+__device__ __forceinline__ uint32_t members64(double d2, const Cuts<NQ>& C) {
   uint32_t m = 0;
 #pragma unroll
   for (int g = 0; g < NQ; ++g)
@@ -471,7 +471,7 @@ __device__ inline uint32_t members64(double d2, const Cuts<NQ>& C) {
 // below lo (inside) or above hi (outside); `unsure` when any needed
 // threshold is within its band, then the FP64 expression decides.
 template <int NQ>
-__device__ inline uint32_t members32(float e2, const Cuts<NQ>& C, bool* unsure) {
+__device__ __forceinline__ uint32_t members32(float e2, const Cuts<NQ>& C, bool* unsure) {
   uint32_t m = 0;
   bool u = false;
 #pragma unroll
@@ -493,7 +493,7 @@ __device__ inline uint32_t members32(float e2, const Cuts<NQ>& C, bool* unsure) 
 // list bits.  `cur` holds the wave's running counts (count pass) or write
 // cursors (fill pass).
 template <bool FILL, int NQ>
-__device__ inline void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
+__device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
                                  int64_t gj, int sx, int sy, int sz, double dd, int64_t* cur) {
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
@@ -516,7 +516,7 @@ __device__ inline void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t 
 }
 
 template <bool FILL, int NQ>
-__device__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
+__device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
                                 int mode, int64_t* cur) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
@@ -570,12 +570,12 @@ __device__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysD
   }
 }
 
-__device__ inline double readlane_d(double v, int l) {
+__device__ __forceinline__ double readlane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
   return __hiloint2double(hi, lo);
 }
-__device__ inline float readlane_f(float v, int l) {
+__device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
@@ -596,7 +596,7 @@ __device__ inline float readlane_f(float v, int l) {
 // search.rs:695-697); a self image (same atom, bs != 0) is written for both
 // S and -S, as the reference emits both (search.rs:549).
 template <bool FILL, int NQ>
-__device__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
+__device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
                           int part, WaveLds& L, int64_t* cur) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
@@ -621,7 +621,12 @@ __device__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d
   const int t_lo = tc + part * slice;
   const int t_hi = min(T, t_lo + slice);
   const bool fp32 = A.use_fp32;
-  const float rej32 = C.hi[C.cnt - 1];  // beyond the group's largest band: no list
+  // beyond the group's largest band: no list (unrolled so that C stays in
+  // registers; a runtime index would put it in scratch memory)
+  float rej32 = -INFINITY;
+#pragma unroll
+  for (int g = 0; g < NQ; ++g)
+    if (C.has(g)) rej32 = fmaxf(rej32, C.hi[g]);
   for (int hb = hs; hb < he; hb += 64) {
     const int hn = min(64, he - hb);
     Pos64 hp = {0.0, 0.0, 0.0, 0.0};

@@ -84,6 +84,7 @@ _SIGS = {
     "nl_device_free": (C.c_int, [C.c_void_p]),
     "nl_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "nl_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "nl_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "nl_device_synchronize": (C.c_int, []),
     "nl_set_device": (C.c_int, [C.c_int]),
     "nl_device_count": (C.c_int, []),
