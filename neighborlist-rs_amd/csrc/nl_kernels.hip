@@ -864,25 +864,33 @@ __global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
   uint64_t pre = winc - sum;
   for (int i = 0; i < w; ++i) pre += s_w[i];
   const uint64_t tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-  if (tid == 0) {
-    uint64_t* st = status + tile;
+  // decoupled look-back by the first wavefront: 64 predecessors per probe
+  if (w == 0) {
     uint64_t excl = 0;
     if (tile == 0) {
-      atomic_store_agent(st, kFlagPre64 | tot);
+      if (lane == 0) atomic_store_agent(status, kFlagPre64 | tot);
     } else {
-      atomic_store_agent(st, kFlagAgg64 | tot);
-      int64_t t = (int64_t)tile - 1;
+      if (lane == 0) atomic_store_agent(status + tile, kFlagAgg64 | tot);
+      int64_t t_hi = (int64_t)tile - 1;  // newest predecessor of this window
       while (true) {
-        const uint64_t x = atomic_load_agent(status + t);
+        const int64_t t = t_hi - lane;
+        uint64_t x = t >= 0 ? atomic_load_agent(status + t) : kFlagPre64;  // before 0: prefix 0
         const uint64_t flag = x & ~kValMask64;
-        if (flag == 0) continue;
-        excl += x & kValMask64;
-        if (flag == kFlagPre64) break;
-        --t;
+        const uint64_t pre_mask = __ballot(flag == kFlagPre64);
+        const uint64_t zero_mask = __ballot(flag == 0);
+        // lanes up to and including the nearest inclusive prefix must be ready
+        const int first_pre = pre_mask ? __builtin_ctzll(pre_mask) : 64;
+        const uint64_t need = first_pre >= 63 ? ~0ull : ((1ull << (first_pre + 1)) - 1);
+        if (zero_mask & need) continue;  // a predecessor is still ranking: probe again
+        uint64_t v = (lane <= first_pre) ? (x & kValMask64) : 0ull;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        excl += v;
+        if (first_pre < 64) break;
+        t_hi -= 64;
       }
-      atomic_store_agent(st, kFlagPre64 | (excl + tot));
+      if (lane == 0) atomic_store_agent(status + tile, kFlagPre64 | (excl + tot));
     }
-    s_excl = excl;
+    if (lane == 0) s_excl = excl;
   }
   __syncthreads();
   uint64_t run = s_excl + pre;
