@@ -93,6 +93,37 @@ int nl_plan_copy(const nl_plan* p, size_t list, int64_t* edge_i, int64_t* edge_j
 int nl_plan_stats(const nl_plan* p, int64_t* replays, int64_t* recaptures, int64_t* capacity);
 void nl_plan_free(nl_plan* p);
 
+/* ---- serving: continuous batching -----------------------------------------
+ * No counterpart in the reference (its API is synchronous, one call per
+ * request); this is the serving form of nl_search_single with one cutoff.
+ * Requests are submitted from any thread and return an id at once.  A worker
+ * thread keeps up to `slots` ragged batches in flight, each on its own HIP
+ * stream with its own buffers; a free slot takes the queued requests in
+ * arrival order, up to max_atoms atoms and max_systems systems per batch,
+ * while the estimated edges of all batches on the device stay within
+ * edge_budget (a request alone is always admitted).  Each request keeps its
+ * own cell, periodicity and cutoff, its lists use its own atom indices, its
+ * edges are nl_search_single's for the same input, and its arrays (order
+ * included) are the same whatever batch it ran in.  nl_server_wait: 0 done, 1 timed out (timeout_s >= 0), nonzero
+ * with nl_last_error on failure.  nl_server_take copies the lists (any of
+ * the outputs may be NULL) and forgets the request. */
+typedef struct nl_server nl_server;
+nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
+                            int64_t edge_budget);
+int64_t nl_server_submit(nl_server* s, const double* positions, size_t n, const double* cell,
+                         const uint8_t* pbc, double cutoff);
+int nl_server_wait(nl_server* s, int64_t id, double timeout_s);
+int nl_server_result_info(nl_server* s, int64_t id, int64_t* n_edges, double* latency_s,
+                          int* batch_size);
+int nl_server_take(nl_server* s, int64_t id, int64_t* edge_i, int64_t* edge_j,
+                   int32_t* shifts);
+/* counters since creation; refills = batches whose edge estimate was short
+ * (their fill pass ran a second time into larger buffers) */
+int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t* max_inflight,
+                    int64_t* max_batch, int64_t* refills);
+/* waits for every submitted request, stops the worker, frees the buffers */
+void nl_server_free(nl_server* s);
+
 /* Host-side cell helpers with the reference's arithmetic (no GPU needed):
  * H^-1 by the cofactor formula of Cell::new (cell.rs:18-21; returns nonzero
  * with "Cell matrix is not invertible" when det == 0) and Cell::wrap

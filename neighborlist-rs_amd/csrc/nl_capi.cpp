@@ -9,6 +9,7 @@
 #include "../../include/nl_api.h"
 #include "../../include/nl_engine_ext.h"
 #include "nl_engine.h"
+#include "nl_server.h"
 
 using nl::Engine;
 using nl::Request;
@@ -22,6 +23,10 @@ struct nl_plan {
   nl::Plan p;
   nl::Engine* eng;
   int64_t cap;
+};
+
+struct nl_server {
+  nl::Server* s;
 };
 
 namespace {
@@ -541,6 +546,72 @@ int nl_memcpy_d2d(void* dst, const void* src, size_t bytes) {
   return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
 }
 int nl_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
+                            int64_t edge_budget) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    nl::set_error("no HIP device available (libnlmi355 runs on the GPU only)");
+    return nullptr;
+  }
+  if (slots <= 0 || max_atoms <= 0 || max_systems <= 0 || edge_budget <= 0) {
+    nl::set_error("server limits must be positive");
+    return nullptr;
+  }
+  return new nl_server{new nl::Server(nl::engine_for_current_device(), slots, max_atoms,
+                                      max_systems, edge_budget)};
+}
+
+int64_t nl_server_submit(nl_server* s, const double* positions, size_t n, const double* cell,
+                         const uint8_t* pbc, double cutoff) {
+  if (!s || !positions) {
+    nl::set_error("server or positions pointer is NULL");
+    return -1;
+  }
+  return s->s->submit(positions, (int64_t)n, cell, pbc, cutoff);
+}
+
+int nl_server_wait(nl_server* s, int64_t id, double timeout_s) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  return s->s->wait(id, timeout_s);
+}
+
+int nl_server_result_info(nl_server* s, int64_t id, int64_t* n_edges, double* latency_s,
+                          int* batch_size) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  return s->s->info(id, n_edges, latency_s, batch_size);
+}
+
+int nl_server_take(nl_server* s, int64_t id, int64_t* edge_i, int64_t* edge_j,
+                   int32_t* shifts) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  return s->s->take(id, edge_i, edge_j, shifts);
+}
+
+int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t* max_inflight,
+                    int64_t* max_batch, int64_t* refills) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  s->s->stats(completed, batches, max_inflight, max_batch, refills);
+  return 0;
+}
+
+void nl_server_free(nl_server* s) {
+  if (!s) return;
+  delete s->s;
+  delete s;
+}
 
 int nl_set_device(int device) {
   hipError_t e = hipSetDevice(device);

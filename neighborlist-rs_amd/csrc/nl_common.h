@@ -31,6 +31,8 @@ struct alignas(16) SysDesc {
   double hinv[9];   // row-major H^-1 (reference cofactor formula)
   double w[3];      // perpendicular widths 1/|row_k(H^-1)|
   double delta;     // FP32 filter band half-width (A), cell-list path
+  double rc_max;    // largest cutoff of this system (all systems share it
+                    // unless the request gives per-system cutoffs)
   float h32[9];     // H rounded to FP32 for the filter
   int64_t atom_start;
   int64_t bin_base;  // first global bin id of this system
@@ -56,7 +58,7 @@ struct Params {
   int32_t cell_div;    // GPU bin edge = rc_max / cell_div
   int32_t use_fp32;    // FP32 filter + FP64 band recheck on the cell-list path
   int32_t global_index;// add atom_start to emitted indices (batch API)
-  int32_t pad;
+  int32_t per_sys_rc;  // rc2 holds [n_sys][k] squared cutoffs instead of [k]
   double rc_max;       // max cutoff (NaN-ignoring fold, single.rs:106)
 };
 

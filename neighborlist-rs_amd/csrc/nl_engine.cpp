@@ -63,8 +63,17 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   for (int q = 0; q < K; ++q) order[q] = q;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return req.cutoffs[a] < req.cutoffs[b]; });
-  rc2.resize(K);
-  for (int q = 0; q < K; ++q) rc2[q] = req.cutoffs[order[q]] * req.cutoffs[order[q]];
+  per_sys = !req.sys_cutoff.empty();
+  if (per_sys && (K != 1 || (int)req.sys_cutoff.size() != S)) {
+    set_error("per-system cutoffs need one list and one cutoff per system");
+    return 1;
+  }
+  rc2.resize(per_sys ? S : K);
+  if (per_sys) {
+    for (int s = 0; s < S; ++s) rc2[s] = req.sys_cutoff[s] * req.sys_cutoff[s];
+  } else {
+    for (int q = 0; q < K; ++q) rc2[q] = req.cutoffs[order[q]] * req.cutoffs[order[q]];
+  }
   double rc_max = NAN;  // NaN-ignoring fold (single.rs:106)
   for (double c : req.cutoffs) rc_max = std::isnan(rc_max) ? c : std::fmax(rc_max, c);
 
@@ -77,6 +86,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
     d.atom_start = req.starts[s];
     d.n_atoms = (int32_t)(req.starts[s + 1] - req.starts[s]);
     d.has_cell = req.sys[s].has_cell;
+    d.rc_max = per_sys ? req.sys_cutoff[s] : rc_max;
     for (int k = 0; k < 3; ++k) d.pbc[k] = req.sys[s].has_cell ? (req.sys[s].pbc[k] != 0) : 0;
     if (d.has_cell) {
       std::memcpy(d.h, req.sys[s].h, sizeof d.h);
@@ -124,6 +134,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   prm.cell_div = tune.cell_div;
   prm.use_fp32 = tune.use_fp32;
   prm.global_index = req.global_index;
+  prm.per_sys_rc = per_sys;
   prm.rc_max = rc_max;
   return 0;
 }
@@ -134,7 +145,7 @@ std::vector<size_t> Pipeline::buffer_sizes() const {
       sizeof(SysDesc) * S,                                          // sys
       8 * (size_t)(S + 1),                                          // start
       8 * (size_t)(S + 1),                                          // items
-      8 * (size_t)K,                                                // rc2
+      8 * rc2.size(),                                               // rc2
       sizeof(float2) * (size_t)S * K,                               // band
       4 * (size_t)n, 4 * (size_t)n, 4 * (size_t)n, 4 * (size_t)n,  // keys/vals A/B
       4 * (size_t)z_words,                                          // zero
@@ -175,7 +186,7 @@ void Pipeline::stage_host(char* pin) const {
   std::memcpy(pin, desc.data(), sizeof(SysDesc) * S);
   std::memcpy(pin + off_start(), starts.data(), 8 * (size_t)(S + 1));
   std::memcpy(pin + off_items(), item_base.data(), 8 * (size_t)(S + 1));
-  std::memcpy(pin + off_rc2(), rc2.data(), 8 * (size_t)K);
+  std::memcpy(pin + off_rc2(), rc2.data(), 8 * rc2.size());
 }
 
 const int64_t* Pipeline::host_bounds(const char* pin) const {
@@ -192,7 +203,7 @@ hipError_t Pipeline::enqueue_upload(const Buffers& b, const char* pin, hipStream
   NL_TRY(hipMemcpyAsync(b.sys, pin, sizeof(SysDesc) * S, hipMemcpyHostToDevice, st));
   NL_TRY(hipMemcpyAsync(b.start, pin + off_start(), 8 * (S + 1), hipMemcpyHostToDevice, st));
   NL_TRY(hipMemcpyAsync(b.items, pin + off_items(), 8 * (S + 1), hipMemcpyHostToDevice, st));
-  NL_TRY(hipMemcpyAsync(b.rc2, pin + off_rc2(), 8 * K, hipMemcpyHostToDevice, st));
+  NL_TRY(hipMemcpyAsync(b.rc2, pin + off_rc2(), 8 * rc2.size(), hipMemcpyHostToDevice, st));
   return hipSuccess;
 }
 
@@ -219,6 +230,7 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.disjoint = P.disjoint;
   A.use_fp32 = P.tune.use_fp32;
   A.global_index = P.global_index;
+  A.per_sys_rc = P.per_sys;
   A.group = P.tune.group;
   A.parts = P.tune.parts;
   return A;

@@ -44,6 +44,7 @@ struct Request {
   std::vector<int64_t> starts;  // n_sys + 1
   std::vector<SysIn> sys;
   std::vector<double> cutoffs;  // input order
+  std::vector<double> sys_cutoff;  // optional: one cutoff per system (then cutoffs = {max})
   bool multi = false;
   bool disjoint = false;
   bool global_index = true;
@@ -105,7 +106,7 @@ class Pipeline {
   int S = 0, K = 0, passes = 0;
   Tuning tune;
   Params prm{};
-  bool pos_device = false, global_index = true, disjoint = false;
+  bool pos_device = false, global_index = true, disjoint = false, per_sys = false;
   const double* host_pos = nullptr;
   std::vector<int> order;  // sorted rank -> input list
   std::vector<double> rc2; // sorted
@@ -116,7 +117,7 @@ class Pipeline {
   size_t off_start() const { return sizeof(SysDesc) * S; }
   size_t off_items() const { return off_start() + 8 * (size_t)(S + 1); }
   size_t off_rc2() const { return off_items() + 8 * (size_t)(S + 1); }
-  size_t off_bounds() const { return off_rc2() + 8 * (size_t)K; }
+  size_t off_bounds() const { return off_rc2() + 8 * rc2.size(); }
 };
 
 class Engine;

@@ -19,7 +19,7 @@ struct PairArgs {
   const Pos64* spos64;       // sorted wrapped positions (FP64), w = local index
   const float4* spos32;      // sorted wrapped positions (FP32), w = local index bits
   const int4* sidx;          // sorted wrap shifts s_a, w = local index
-  const double* rc2;         // k squared cutoffs, ascending
+  const double* rc2;         // k squared cutoffs, ascending ([n_sys][k] if per_sys_rc)
   const float2* band;        // [n_sys][k] FP32 accept/reject thresholds
   uint32_t* counts;          // [k][n] (count pass)
   const int64_t* offsets;    // [k][n] + 1 (fill pass)
@@ -30,6 +30,7 @@ struct PairArgs {
   int64_t n;                 // work items
   int64_t capacity;          // edges the output buffers hold (fill pass)
   int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index;
+  int32_t per_sys_rc;        // rc2 is [n_sys][k] (per-system cutoffs)
   int32_t group;             // home atoms per cell-list work item (<= 64)
   int32_t parts;             // slices of the half stencil per home group
 };

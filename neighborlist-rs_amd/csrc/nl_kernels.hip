@@ -110,6 +110,8 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
   SysDesc* d = sys + s;
   const int64_t a0 = d->atom_start;
   const int n = d->n_atoms;
+  const double rc_max = d->rc_max;
+  const double* rc2s = p.per_sys_rc ? rc2 + (int64_t)s * p.k : rc2;
   __shared__ double s_mn[3][4], s_mx[3][4];
   if (!d->has_cell) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
   if (threadIdx.x != 0) return;
   double h[9];
   if (!d->has_cell) {
-    const double margin = p.rc_max + 1.0;  // AUTO_BOX_MARGIN (single.rs:6)
+    const double margin = rc_max + 1.0;  // AUTO_BOX_MARGIN (single.rs:6)
     for (int k = 0; k < 9; ++k) h[k] = 0.0;
     for (int k = 0; k < 3; ++k) {
       double mn = s_mn[k][0], mx = s_mx[k][0];
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
   }
   for (int k = 0; k < 3; ++k) d->w[k] = w[k];
   const double minw = fmin(fmin(w[0], w[1]), w[2]);
-  const bool mic_safe = p.rc_max * 2.0 < minw;
+  const bool mic_safe = rc_max * 2.0 < minw;
   const uint64_t thr = ((uint64_t)(uint32_t)p.bf_threshold_hi << 32) | (uint32_t)p.bf_threshold_lo;
   const bool any_pbc = d->pbc[0] || d->pbc[1] || d->pbc[2];
   int mode;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
     return;
   }
   // GPU bin grid: edge ~ rc_max / cell_div, at most bin_cap bins.
-  const double edge = p.rc_max / (double)p.cell_div;
+  const double edge = rc_max / (double)p.cell_div;
   int nb[3];
   for (int k = 0; k < 3; ++k) {
     double v = floor(w[k] / edge);
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
   for (int k = 0; k < 3; ++k) {
     // covering reach: ceil(rc n / w), widened by one bin when the ratio sits
     // within 1e-9 of an integer (floating-point slack at the boundary)
-    double r = p.rc_max * (double)nb[k] / w[k];
+    double r = rc_max * (double)nb[k] / w[k];
     ns[k] = (int)ceil(r + 1e-9);
     d->nb[k] = nb[k];
     d->ns[k] = ns[k];
@@ -238,10 +240,10 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
     P = fmax(P, pr);
     O = fmax(O, orr);
   }
-  const double delta = 64.0 * 5.9604644775390625e-08 * (P + O + p.rc_max);
+  const double delta = 64.0 * 5.9604644775390625e-08 * (P + O + rc_max);
   d->delta = delta;
   for (int q = 0; q < p.k; ++q) {
-    double rc = sqrt(rc2[q]);
+    double rc = sqrt(rc2s[q]);
     double lo = rc - delta;
     lo = lo > 0.0 ? lo * lo : 0.0;
     double hi = (rc + delta) * (rc + delta);
@@ -802,7 +804,8 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   Cuts<NQ> C;
   C.cnt = A.q_cnt;
   C.disjoint = A.disjoint != 0;
-  C.prev = A.q_lo > 0 ? A.rc2[A.q_lo - 1] : -INFINITY;
+  const double* rc2s = A.per_sys_rc ? A.rc2 + (int64_t)s * A.k : A.rc2;
+  C.prev = A.q_lo > 0 ? rc2s[A.q_lo - 1] : -INFINITY;
   const float2* band = A.band + (int64_t)s * A.k;
   const float2 bp = A.q_lo > 0 ? band[A.q_lo - 1] : make_float2(-INFINITY, -INFINITY);
   C.prev_lo = bp.x;
@@ -812,7 +815,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   for (int g = 0; g < NQ; ++g) {
     const bool on = g < A.q_cnt;
     const int q = A.q_lo + (on ? g : 0);
-    C.r[g] = on ? A.rc2[q] : -INFINITY;
+    C.r[g] = on ? rc2s[q] : -INFINITY;
     const float2 b = band[q];
     C.lo[g] = on ? b.x : -INFINITY;
     C.hi[g] = on ? b.y : -INFINITY;

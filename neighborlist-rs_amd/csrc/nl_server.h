@@ -1,0 +1,53 @@
+// nl_server.h -- continuous-batching request server (see nl_server.cpp).
+#pragma once
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace nl {
+
+class Engine;
+struct SrvReq;
+
+int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc);
+
+class Server {
+ public:
+  Server(Engine* eng, int slots, int64_t max_atoms, int64_t max_systems, int64_t edge_budget);
+  ~Server();
+  // returns the request id (> 0) or -1 with nl_last_error set
+  int64_t submit(const double* pos, int64_t n, const double* h, const uint8_t* pbc,
+                 double cutoff);
+  int wait(int64_t id, double timeout_s);  // 0 done, 1 timed out, -1 failed
+  int info(int64_t id, int64_t* n_edges, double* latency_s, int* batch_size);
+  int take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh);  // copies, then forgets
+  void stats(int64_t* completed, int64_t* batches, int64_t* max_inflight, int64_t* max_batch,
+             int64_t* refills);
+
+ private:
+  struct Slot;
+  void loop();
+  bool launch(Slot& s, std::vector<SrvReq*>& batch);
+  bool on_counted(Slot& s);
+  void deliver(Slot& s);
+  void fail_slot(Slot& s, const std::string& msg);
+
+  Engine* eng_;
+  int64_t max_atoms_, max_systems_, budget_;
+  std::vector<Slot*> slots_;
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<SrvReq*> queue_;
+  std::unordered_map<int64_t, SrvReq*> reqs_;
+  bool stop_ = false;
+  int64_t next_id_ = 1, submitted_ = 0, completed_ = 0, batches_ = 0, max_inflight_ = 0,
+          max_batch_ = 0, inflight_est_ = 0, refills_ = 0;
+};
+
+}  // namespace nl

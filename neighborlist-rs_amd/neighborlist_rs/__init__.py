@@ -394,3 +394,6 @@ def get_stack_threshold():
 
 def set_stack_threshold(val):
     lib.nl_set_stack_threshold(int(val))
+
+
+from .serving import Server  # noqa: E402  (continuous-batching server)

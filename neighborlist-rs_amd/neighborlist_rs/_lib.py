@@ -78,6 +78,14 @@ _SIGS = {
     "nl_plan_copy": (C.c_int, [C.c_void_p, C.c_size_t, i64p, i64p, i32p]),
     "nl_plan_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p]),
     "nl_plan_free": (None, [C.c_void_p]),
+    "nl_server_create": (C.c_void_p, [C.c_int, C.c_int64, C.c_int64, C.c_int64]),
+    "nl_server_submit": (C.c_int64, [C.c_void_p, dp, C.c_size_t, dp, u8p, C.c_double]),
+    "nl_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
+    "nl_server_result_info": (C.c_int, [C.c_void_p, C.c_int64, i64p, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int)]),
+    "nl_server_take": (C.c_int, [C.c_void_p, C.c_int64, i64p, i64p, i32p]),
+    "nl_server_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p, i64p]),
+    "nl_server_free": (None, [C.c_void_p]),
     "nl_cell_inverse": (C.c_int, [dp, dp]),
     "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
     "nl_device_malloc": (C.c_void_p, [C.c_size_t]),

@@ -1,0 +1,96 @@
+"""Continuous-batching request server over libnlmi355.
+
+The reference answers one request per synchronous call
+(src/python_api.rs:139-186, build_neighborlists).  `Server` takes the same
+inputs (a PyCell or None, an (N, 3) float64 array, one cutoff) from any
+number of threads, batches whatever is queued into ragged batches on the GPU
+(several in flight on separate HIP streams) and hands each request back the
+same dict build_neighborlists would return for it.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+
+
+class Server:
+    """submit() returns a request id at once; result(id) blocks for its lists.
+
+    slots        batches in flight at once (each has its own stream and buffers)
+    max_atoms    atoms per batch
+    max_systems  requests per batch
+    edge_budget  estimated edges of all in-flight batches (admission control;
+                 a request alone is always admitted)
+    """
+
+    def __init__(self, slots=3, max_atoms=1 << 20, max_systems=1024, edge_budget=1 << 28):
+        self._p = lib.nl_server_create(int(slots), int(max_atoms), int(max_systems),
+                                       int(edge_budget))
+        if not self._p:
+            raise RuntimeError(_lib.last_error())
+
+    def close(self):
+        if self._p:
+            lib.nl_server_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, cell, positions, cutoff):
+        from . import _cell_args, _positions
+
+        cutoff = float(cutoff)
+        if not np.isfinite(cutoff) or cutoff <= 0.0:
+            raise ValueError("Cutoff must be positive and finite")
+        pos = _positions(positions)
+        h, pb = _cell_args(cell)
+        rid = int(lib.nl_server_submit(self._p, pos.ctypes.data_as(_lib.dp), pos.shape[0], h, pb,
+                                       cutoff))
+        if rid < 0:
+            raise ValueError(_lib.last_error())
+        return rid
+
+    def wait(self, rid, timeout=None):
+        rc = lib.nl_server_wait(self._p, int(rid), -1.0 if timeout is None else float(timeout))
+        if rc == 1:
+            return False
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return True
+
+    def info(self, rid):
+        """(edges, latency in seconds from submit to completion, batch size)"""
+        m = C.c_int64()
+        lat = C.c_double()
+        bs = C.c_int()
+        if lib.nl_server_result_info(self._p, int(rid), C.byref(m), C.byref(lat), C.byref(bs)):
+            raise RuntimeError(_lib.last_error())
+        return int(m.value), float(lat.value), int(bs.value)
+
+    def result(self, rid, timeout=None):
+        """Blocks until done; returns {"edge_index": (2, M) int64, "shift": (M, 3) int32}
+        and forgets the request."""
+        if not self.wait(rid, timeout):
+            raise TimeoutError(f"request {rid} not done after {timeout} s")
+        m, _, _ = self.info(rid)
+        ei = np.empty((2, m), np.int64)
+        sh = np.empty((m, 3), np.int32)
+        if lib.nl_server_take(self._p, int(rid), ei[0].ctypes.data_as(_lib.i64p),
+                              ei[1].ctypes.data_as(_lib.i64p), sh.ctypes.data_as(_lib.i32p)):
+            raise RuntimeError(_lib.last_error())
+        return {"edge_index": ei, "shift": sh}
+
+    def stats(self):
+        v = [C.c_int64() for _ in range(5)]
+        lib.nl_server_stats(self._p, *[C.byref(x) for x in v])
+        return dict(zip(("completed", "batches", "max_inflight", "max_batch", "refills"),
+                        (int(x.value) for x in v)))
