@@ -9,7 +9,9 @@ resident in HBM, outputs left on the device.  `value` = edges/s summed over
 the three lists.  Beside it: systems/s of the same batch through the batched
 path and through the plain path (one call per system), p50/p99 latency of a
 batch build, the 10k-atom triclinic system at rc 6 against the CPU oracle,
-the roofline of the dominant kernel (pair fill) and the CPU baseline.
+the serving trace (continuous batching: capacity, p50/p99 latency at 70% load,
+against the plain path), the roofline of the dominant kernel (pair fill) and
+the CPU baseline.
 
 Prints ONE JSON line.
 """
@@ -80,6 +82,127 @@ def cell_arrays(cells):
     return H9, has, pbc
 
 
+def serve_trace(L, n_req, slots, load):
+    """Serving trace: Poisson arrivals of single systems (N log-uniform in
+    [16, 50000], isolated / orthorhombic / triclinic, rc uniform in [4, 12]).
+
+    Capacity: all requests queued at once (closed loop), results left on the
+    device (the consumer of a served list is a GPU model) and, separately,
+    copied into numpy arrays.  Latency: open loop, Poisson arrivals at `load`
+    times the measured capacity; latency = submit -> lists complete on the
+    device.  Plain: the same requests one call each through the plain path
+    (host input, lists left on the device)."""
+    import threading
+
+    import neighborlist_rs as nr
+    from neighborlist_rs.workloads import c5_serving_trace
+
+    lib = L.lib
+    trace = c5_serving_trace(seed=5, n_requests=n_req)
+    cells = [None if H is None else nr.PyCell(np.asarray(H).tolist(), [True] * 3)
+             for (_, _, H, _) in trace]
+    atoms = sum(p.shape[0] for (_, p, _, _) in trace)
+
+    def plain_pass():
+        tot = 0
+        for (_, p, H, rc) in trace:
+            sz = np.array([p.shape[0]], np.int64)
+            H9, has, pbc = cell_arrays([H])
+            c = np.array([rc])
+            r = C.c_void_p()
+            if lib.nl_search_systems(p.ctypes.data_as(L.dp), p.shape[0], sz.ctypes.data_as(L.i64p),
+                                     1, H9.ctypes.data_as(L.dp), has.ctypes.data_as(L.u8p),
+                                     pbc.ctypes.data_as(L.u8p), c.ctypes.data_as(L.dp), 1, 0, 0,
+                                     0, 0, 0, C.byref(r)) != 0:
+                raise RuntimeError(L.last_error())
+            tot += int(lib.nl_result_num_edges(r, 0))
+            lib.nl_result_free(r)
+        return tot
+
+    plain_pass()
+    t0 = time.perf_counter()
+    plain_edges = plain_pass()
+    plain_s = time.perf_counter() - t0
+
+    srv = nr.Server(slots=slots, max_atoms=1 << 18, max_systems=256, edge_budget=1 << 27)
+
+    def closed(host):
+        t0 = time.perf_counter()
+        ids = [srv.submit(c, p, rc) for c, (_, p, _, rc) in zip(cells, trace)]
+        tot = 0
+        for rid in ids:
+            if host:
+                tot += srv.result(rid)["shift"].shape[0]
+            else:
+                m, _, _, _ = srv.result_device(rid)
+                srv.release(rid)
+                tot += m
+        return time.perf_counter() - t0, tot
+
+    closed(False)  # warm-up: grows slot buffers and the output pool
+    cap_s, served_edges = closed(False)
+    assert served_edges == plain_edges, (served_edges, plain_edges)
+    host_s, host_edges = closed(True)
+    assert host_edges == plain_edges
+    capacity = n_req / cap_s
+    log(f"serving: plain {n_req / plain_s:.1f} sys/s, served capacity {capacity:.1f} sys/s "
+        f"(device lists), {n_req / host_s:.1f} sys/s (numpy lists)")
+
+    # open loop at `load` x capacity
+    qps = load * capacity
+    rng = np.random.default_rng(55)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n_req))
+    ids = [None] * n_req
+    ready = threading.Semaphore(0)
+
+    def producer():
+        t0 = time.perf_counter()
+        for k, (c, (_, p, _, rc)) in enumerate(zip(cells, trace)):
+            dt = arrivals[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids[k] = srv.submit(c, p, rc)
+            ready.release()
+
+    st0 = srv.stats()
+    th = threading.Thread(target=producer)
+    t_start = time.perf_counter()
+    th.start()
+    lat, bsz = [], []
+    for k in range(n_req):
+        ready.acquire()
+        srv.wait(ids[k])
+        _, l, b = srv.info(ids[k])
+        lat.append(l)
+        bsz.append(b)
+        srv.release(ids[k])
+    t_end = time.perf_counter()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    return {
+        "workload": f"c5_serving_trace: {n_req} requests, {atoms} atoms, {plain_edges} edges; "
+                    "N log-uniform [16,50000], rc U[4,12], 1/3 isolated, 1/3 orthorhombic, "
+                    "1/3 triclinic",
+        "slots": slots,
+        "capacity_systems_per_s": round(capacity, 1),
+        "capacity_edges_per_s": round(plain_edges / cap_s, 1),
+        "numpy_delivery_systems_per_s": round(n_req / host_s, 1),
+        "plain_gpu_systems_per_s": round(n_req / plain_s, 1),
+        "served_over_plain": round(capacity / (n_req / plain_s), 2),
+        "open_loop": {"offered_qps": round(qps, 1),
+                      "sustained_systems_per_s": round(n_req / (t_end - t_start), 1),
+                      "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                                     "p99": round(1e3 * float(np.percentile(lat, 99)), 3),
+                                     "mean": round(1e3 * float(lat.mean()), 3)},
+                      "mean_batch": round(float(np.mean(bsz)), 2),
+                      "batches": st["batches"] - st0["batches"],
+                      "max_inflight": st["max_inflight"]},
+        "refills": st["refills"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +214,11 @@ def main():
                     help="'auto' (autotuner) or cell_div,group,parts,fp32 pinned")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the timed batch steps (for rocprofv3)")
+    ap.add_argument("--serve-requests", type=int, default=300,
+                    help="requests in the serving trace (0: skip serving)")
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="open-loop arrival rate as a fraction of the served capacity")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a device-to-device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -229,6 +357,10 @@ def main():
     c3_ms = 1e3 * (time.perf_counter() - t0) / k3
     log(f"c3 rc6: {c3_ms:.3f} ms, {e3} edges")
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_trace(L, args.serve_requests, args.serve_slots, args.serve_load)
+
     cpu = None
     c3_cpu_ms = None
     if not args.no_cpu:
@@ -289,6 +421,7 @@ def main():
                              "cpu_ms": None if c3_cpu_ms is None else round(c3_cpu_ms, 3),
                              "speedup_vs_cpu": None if c3_cpu_ms is None
                              else round(c3_cpu_ms / c3_ms, 2)},
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -104,9 +104,15 @@ void nl_plan_free(nl_plan* p);
  * edge_budget (a request alone is always admitted).  Each request keeps its
  * own cell, periodicity and cutoff, its lists use its own atom indices, its
  * edges are nl_search_single's for the same input, and its arrays (order
- * included) are the same whatever batch it ran in.  nl_server_wait: 0 done, 1 timed out (timeout_s >= 0), nonzero
- * with nl_last_error on failure.  nl_server_take copies the lists (any of
- * the outputs may be NULL) and forgets the request. */
+ * included) are the same whatever batch it ran in.  Results stay in device
+ * memory until taken or released.  nl_server_wait: 0 done, 1 timed out
+ * (timeout_s >= 0), nonzero with nl_last_error on failure.  nl_server_take
+ * copies the lists to host memory (any of the outputs may be NULL) and
+ * releases the request; nl_server_result_device gives their device
+ * addresses instead (edge_i and edge_j: M int64 each, shifts: M x 3 int32),
+ * valid until nl_server_release, which the caller issues once its own device
+ * work on them has completed.  The edge budget covers the batches on the
+ * device, not results held by callers. */
 typedef struct nl_server nl_server;
 nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
                             int64_t edge_budget);
@@ -117,6 +123,9 @@ int nl_server_result_info(nl_server* s, int64_t id, int64_t* n_edges, double* la
                           int* batch_size);
 int nl_server_take(nl_server* s, int64_t id, int64_t* edge_i, int64_t* edge_j,
                    int32_t* shifts);
+int nl_server_result_device(nl_server* s, int64_t id, const int64_t** edge_i,
+                            const int64_t** edge_j, const int32_t** shifts);
+int nl_server_release(nl_server* s, int64_t id);
 /* counters since creation; refills = batches whose edge estimate was short
  * (their fill pass ran a second time into larger buffers) */
 int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t* max_inflight,

@@ -597,6 +597,23 @@ int nl_server_take(nl_server* s, int64_t id, int64_t* edge_i, int64_t* edge_j,
   return s->s->take(id, edge_i, edge_j, shifts);
 }
 
+int nl_server_result_device(nl_server* s, int64_t id, const int64_t** edge_i,
+                            const int64_t** edge_j, const int32_t** shifts) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  return s->s->device_view(id, edge_i, edge_j, shifts);
+}
+
+int nl_server_release(nl_server* s, int64_t id) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  return s->s->release(id);
+}
+
 int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t* max_inflight,
                     int64_t* max_batch, int64_t* refills) {
   if (!s) {

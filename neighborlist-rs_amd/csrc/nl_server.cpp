@@ -4,33 +4,30 @@
 // A worker thread keeps up to `slots` batches in flight, each on its own HIP
 // stream with its own device buffers.  Whenever a slot is free it admits the
 // queued requests in arrival order into one ragged batch (per-system cutoffs,
-// local indices), bounded by atoms per batch and by an edge-memory budget:
-// every request carries an estimate of its edge count, and the sum over the
-// batches on the device may not exceed the budget (a request alone always
-// fits, so nothing starves).  Each slot runs the same launch sequence as the
-// plain path (plan, bin, radix sort, finalize, count, scan, fill), so a
-// result is identical whether a request ran alone or inside a batch.
+// local indices), bounded by atoms and systems per batch and by an edge
+// budget: every request carries an estimate of its edge count, and the sum
+// over the batches on the device may not exceed the budget (a request alone
+// is always admitted, so nothing starves).  Each slot runs the same launch
+// sequence as the plain path (plan, bin, radix sort, finalize, count, scan,
+// fill), so a result is the same whether a request ran alone or in a batch.
 //
-// Per slot the worker drives a small state machine on events, never
-// blocking on one slot while others run:
-//   launched -> (counts known) -> [refill if the estimate was short] ->
-//   copying (lists to pinned host memory) -> delivered
+// Results stay on the device: a batch writes into an output block (edges
+// and shifts of all its requests), and each request of the batch holds a
+// reference to the block until the caller takes (copies out) or releases
+// it.  Blocks are recycled through a pool, so the steady state allocates
+// nothing.  Per slot the worker drives a small state machine on events and
+// never blocks on one slot while others run:
+//   launched -> counts known -> [refill if the estimate was short] -> done
+#include "nl_server.h"
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <condition_variable>
 #include <cstring>
-#include <deque>
-#include <memory>
-#include <mutex>
-#include <thread>
-#include <unordered_map>
-#include <vector>
 
 #include "nl_engine.h"
-#include "nl_server.h"
 
 namespace nl {
 
@@ -64,6 +61,16 @@ int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc)
   return (int64_t)(0.5 * (double)n * per_atom) + 16;
 }
 
+// Device output of one batch: int64 edges [2][total] then int32 shifts [total][3].
+struct OutBlock {
+  char* dev = nullptr;
+  int64_t cap = 0;    // edges the block can hold
+  int64_t total = 0;  // edges of the batch written into it
+  int refs = 0;
+  int64_t* edges() const { return (int64_t*)dev; }
+  int32_t* shifts() const { return (int32_t*)(dev + 16 * (size_t)cap); }
+};
+
 struct SrvReq {
   int64_t id = 0;
   std::vector<double> pos;
@@ -73,18 +80,18 @@ struct SrvReq {
   double cutoff = 0;
   double t_submit = 0, t_done = 0;
   int64_t est = 0;
-  int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  int state = 0;  // 0 queued, 1 running, 2 done, 3 failed, 4 being copied out
   std::string err;
-  std::vector<int64_t> ei, ej;
-  std::vector<int32_t> sh;
-  int batch = -1, batch_size = 0, slot = -1;
+  OutBlock* blk = nullptr;
+  int64_t first = 0, len = 0;  // edge range within the block
+  int batch_size = 0;
 };
 
 struct Server::Slot {
   int index = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
-  int state = 0;  // 0 free, 1 launched, 2 refilled, 3 copying
+  int state = 0;  // 0 free, 1 launched, 2 refilling
   Pipeline P;
   Buffers B;
   std::vector<void*> bufs;
@@ -97,23 +104,17 @@ struct Server::Slot {
   size_t hranges_cap = 0;
   int64_t* dranges = nullptr;
   size_t dranges_cap = 0;
-  void* edges = nullptr;
-  void* shifts = nullptr;
-  int64_t out_cap = 0;
-  char* hout = nullptr;
-  size_t hout_cap = 0;
-  int64_t cap_used = 0, total = 0;
+  OutBlock* blk = nullptr;
+  int64_t est_used = 0;
   std::vector<SrvReq*> reqs;
-  int batch = -1;
 };
 
-template <typename T>
-static bool grow_pinned(T** p, size_t* cap, size_t bytes) {
+static bool grow_pinned(void** p, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return true;
   if (*p) (void)hipHostFree(*p);
   *p = nullptr;
-  size_t c = std::max(bytes, *cap * 2);
-  if (hipHostMalloc((void**)p, c, hipHostMallocDefault) != hipSuccess) {
+  const size_t c = std::max(bytes, *cap * 2);
+  if (hipHostMalloc(p, c, hipHostMallocDefault) != hipSuccess) {
     *cap = 0;
     return false;
   }
@@ -121,13 +122,12 @@ static bool grow_pinned(T** p, size_t* cap, size_t bytes) {
   return true;
 }
 
-template <typename T>
-static bool grow_dev(T** p, size_t* cap, size_t bytes) {
+static bool grow_dev(void** p, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return true;
   if (*p) (void)hipFree(*p);
   *p = nullptr;
-  size_t c = std::max(bytes, *cap + *cap / 2);
-  if (hipMalloc((void**)p, c) != hipSuccess) {
+  const size_t c = std::max(bytes, *cap + *cap / 2);
+  if (hipMalloc(p, c) != hipSuccess) {
     *cap = 0;
     return false;
   }
@@ -159,22 +159,28 @@ Server::~Server() {
   }
   cv_.notify_all();
   if (th_.joinable()) th_.join();
+  (void)hipSetDevice(eng_->device());
+  for (Slot* s : slots_) (void)hipStreamSynchronize(s->st);
   for (Slot* s : slots_) {
-    (void)hipStreamSynchronize(s->st);
     for (void* p : s->bufs)
       if (p) (void)hipFree(p);
     if (s->dranges) (void)hipFree(s->dranges);
-    if (s->edges) (void)hipFree(s->edges);
-    if (s->shifts) (void)hipFree(s->shifts);
     if (s->pin) (void)hipHostFree(s->pin);
     if (s->hpos) (void)hipHostFree(s->hpos);
     if (s->hranges) (void)hipHostFree(s->hranges);
-    if (s->hout) (void)hipHostFree(s->hout);
     (void)hipEventDestroy(s->ev);
     (void)hipStreamDestroy(s->st);
     delete s;
   }
-  for (auto& kv : reqs_) delete kv.second;
+  std::vector<OutBlock*> blocks = pool_;
+  for (auto& kv : reqs_) {
+    if (kv.second->blk && --kv.second->blk->refs == 0) blocks.push_back(kv.second->blk);
+    delete kv.second;
+  }
+  for (OutBlock* b : blocks) {
+    (void)hipFree(b->dev);
+    delete b;
+  }
 }
 
 int64_t Server::submit(const double* pos, int64_t n, const double* h, const uint8_t* pbc,
@@ -208,9 +214,17 @@ int64_t Server::submit(const double* pos, int64_t n, const double* h, const uint
   r->t_submit = now_s();
   reqs_[r->id] = r;
   queue_.push_back(r);
-  ++submitted_;
   cv_.notify_all();
   return r->id;
+}
+
+SrvReq* Server::find_done(int64_t id) {
+  auto it = reqs_.find(id);
+  if (it == reqs_.end() || it->second->state != 2) {
+    set_error("request is not complete");
+    return nullptr;
+  }
+  return it->second;
 }
 
 int Server::wait(int64_t id, double timeout_s) {
@@ -228,6 +242,8 @@ int Server::wait(int64_t id, double timeout_s) {
     return 1;
   if (r->state == 3) {
     set_error(r->err);
+    reqs_.erase(id);
+    delete r;
     return -1;
   }
   return 0;
@@ -235,35 +251,69 @@ int Server::wait(int64_t id, double timeout_s) {
 
 int Server::info(int64_t id, int64_t* n_edges, double* latency_s, int* batch_size) {
   std::lock_guard<std::mutex> g(mu_);
-  auto it = reqs_.find(id);
-  if (it == reqs_.end() || it->second->state != 2) {
-    set_error("request is not complete");
-    return -1;
-  }
-  SrvReq* r = it->second;
-  if (n_edges) *n_edges = (int64_t)r->ei.size();
+  SrvReq* r = find_done(id);
+  if (!r) return -1;
+  if (n_edges) *n_edges = r->len;
   if (latency_s) *latency_s = r->t_done - r->t_submit;
   if (batch_size) *batch_size = r->batch_size;
   return 0;
 }
 
+int Server::device_view(int64_t id, const int64_t** ei, const int64_t** ej, const int32_t** sh) {
+  std::lock_guard<std::mutex> g(mu_);
+  SrvReq* r = find_done(id);
+  if (!r) return -1;
+  const OutBlock* b = r->blk;
+  if (ei) *ei = b ? b->edges() + r->first : nullptr;
+  if (ej) *ej = b ? b->edges() + b->total + r->first : nullptr;
+  if (sh) *sh = b ? b->shifts() + 3 * r->first : nullptr;
+  return 0;
+}
+
+void Server::drop_locked(SrvReq* r) {
+  reqs_.erase(r->id);
+  if (r->blk && --r->blk->refs == 0) pool_.push_back(r->blk);
+  delete r;
+}
+
+int Server::release(int64_t id) {
+  std::lock_guard<std::mutex> g(mu_);
+  SrvReq* r = find_done(id);
+  if (!r) return -1;
+  drop_locked(r);
+  return 0;
+}
+
 int Server::take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh) {
+  const int64_t *di = nullptr, *dj = nullptr;
+  const int32_t* ds = nullptr;
+  int64_t m = 0;
   SrvReq* r;
   {
     std::lock_guard<std::mutex> g(mu_);
-    auto it = reqs_.find(id);
-    if (it == reqs_.end() || it->second->state != 2) {
-      set_error("request is not complete");
-      return -1;
+    r = find_done(id);
+    if (!r) return -1;
+    m = r->len;
+    r->state = 4;  // being copied out: no second take or release meanwhile
+    if (r->blk) {
+      di = r->blk->edges() + r->first;
+      dj = r->blk->edges() + r->blk->total + r->first;
+      ds = r->blk->shifts() + 3 * r->first;
     }
-    r = it->second;
-    reqs_.erase(it);
   }
-  const size_t m = r->ei.size();
-  if (ei && m) std::memcpy(ei, r->ei.data(), 8 * m);
-  if (ej && m) std::memcpy(ej, r->ej.data(), 8 * m);
-  if (sh && m) std::memcpy(sh, r->sh.data(), 12 * m);
-  delete r;
+  hipError_t e = hipSuccess;
+  if (m > 0) {
+    (void)hipSetDevice(eng_->device());
+    if (ei && e == hipSuccess) e = hipMemcpy(ei, di, 8 * (size_t)m, hipMemcpyDeviceToHost);
+    if (ej && e == hipSuccess) e = hipMemcpy(ej, dj, 8 * (size_t)m, hipMemcpyDeviceToHost);
+    if (sh && e == hipSuccess) e = hipMemcpy(sh, ds, 12 * (size_t)m, hipMemcpyDeviceToHost);
+  }
+  std::lock_guard<std::mutex> g(mu_);
+  drop_locked(r);
+  if (e != hipSuccess) {
+    set_error(std::string("HIP error copying a served result: ") + hipGetErrorString(e));
+    return -1;
+  }
   return 0;
 }
 
@@ -278,24 +328,63 @@ void Server::stats(int64_t* completed, int64_t* batches, int64_t* max_inflight,
 }
 
 void Server::fail_slot(Slot& s, const std::string& msg) {
+  (void)hipStreamSynchronize(s.st);  // the block may still be written to
   std::lock_guard<std::mutex> g(mu_);
   for (SrvReq* r : s.reqs) {
     r->state = 3;
     r->err = msg;
     r->t_done = now_s();
   }
-  inflight_est_ -= s.cap_used;
+  if (s.blk) pool_.push_back(s.blk);
+  s.blk = nullptr;
+  inflight_est_ -= s.est_used;
+  s.est_used = 0;
   s.reqs.clear();
   s.state = 0;
   done_cv_.notify_all();
 }
 
+// An output block for at least `cap` edges: the smallest pooled block that
+// fits, else a new allocation (when the pool already holds more than a few
+// blocks, its smallest one is freed first).
+OutBlock* Server::get_block(int64_t cap) {
+  OutBlock* victim = nullptr;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    size_t bi = pool_.size();
+    for (size_t i = 0; i < pool_.size(); ++i)
+      if (pool_[i]->cap >= cap && (bi == pool_.size() || pool_[i]->cap < pool_[bi]->cap)) bi = i;
+    if (bi < pool_.size()) {
+      OutBlock* b = pool_[bi];
+      pool_.erase(pool_.begin() + bi);
+      return b;
+    }
+    if (pool_.size() > slots_.size() + 2) {
+      auto it = std::min_element(pool_.begin(), pool_.end(),
+                                 [](OutBlock* a, OutBlock* b) { return a->cap < b->cap; });
+      victim = *it;
+      pool_.erase(it);
+    }
+  }
+  if (victim) {
+    (void)hipFree(victim->dev);
+    delete victim;
+  }
+  OutBlock* b = new OutBlock();
+  b->cap = cap;
+  if (hipMalloc((void**)&b->dev, 28 * (size_t)cap) != hipSuccess) {
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
 // Launch one batch on a free slot (called by the worker, queue lock not held).
 bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
-  (void)hipSetDevice(eng_->device());
   Request req;
   int64_t n = 0;
   req.starts.push_back(0);
+  double rcmax = 0;
   for (SrvReq* r : batch) {
     n += (int64_t)(r->pos.size() / 3);
     req.starts.push_back(n);
@@ -305,9 +394,8 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
     std::memcpy(si.pbc, r->pbc, 3);
     req.sys.push_back(si);
     req.sys_cutoff.push_back(r->cutoff);
+    rcmax = std::max(rcmax, r->cutoff);
   }
-  double rcmax = 0;
-  for (SrvReq* r : batch) rcmax = std::max(rcmax, r->cutoff);
   req.cutoffs = {rcmax};
   req.n = n;
   req.pos_device = true;
@@ -332,10 +420,10 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
     }
   Pipeline::assign(s.B, s.bufs);
   const size_t S = batch.size();
-  if (!grow_pinned(&s.pin, &s.pin_cap, s.P.pinned_bytes()) ||
-      !grow_pinned(&s.hpos, &s.hpos_cap, sizeof(double) * 3 * (size_t)n) ||
-      !grow_pinned(&s.hranges, &s.hranges_cap, 16 * S) ||
-      !grow_dev(&s.dranges, &s.dranges_cap, 16 * S)) {
+  if (!grow_pinned((void**)&s.pin, &s.pin_cap, s.P.pinned_bytes()) ||
+      !grow_pinned((void**)&s.hpos, &s.hpos_cap, sizeof(double) * 3 * (size_t)n) ||
+      !grow_pinned((void**)&s.hranges, &s.hranges_cap, 16 * S) ||
+      !grow_dev((void**)&s.dranges, &s.dranges_cap, 16 * S)) {
     fail_slot(s, "out of memory in the server");
     return false;
   }
@@ -347,24 +435,17 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   s.P.stage_host(s.pin);
   int64_t est = 0;
   for (SrvReq* r : batch) est += r->est;
-  s.cap_used = est + est / 4 + 4096;
-  if (s.out_cap < s.cap_used) {
-    if (s.edges) (void)hipFree(s.edges);
-    if (s.shifts) (void)hipFree(s.shifts);
-    s.edges = s.shifts = nullptr;
-    if (hipMalloc(&s.edges, 16 * (size_t)s.cap_used) != hipSuccess ||
-        hipMalloc(&s.shifts, 12 * (size_t)s.cap_used) != hipSuccess) {
-      s.out_cap = 0;
-      fail_slot(s, "out of device memory for the batch's edge lists");
-      return false;
-    }
-    s.out_cap = s.cap_used;
+  s.blk = get_block(est + est / 4 + 4096);
+  if (!s.blk) {
+    fail_slot(s, "out of device memory for the batch's edge lists");
+    return false;
   }
   hipError_t e = hipMemcpyAsync(s.B.pos, s.hpos, sizeof(double) * 3 * (size_t)n,
                                 hipMemcpyHostToDevice, s.st);
   if (e == hipSuccess) e = s.P.enqueue_upload(s.B, s.pin, s.st);
   if (e == hipSuccess) e = s.P.enqueue_build_count(s.B, s.st);
-  if (e == hipSuccess) e = s.P.enqueue_fill(s.B, s.edges, s.shifts, nullptr, s.out_cap, s.st);
+  if (e == hipSuccess)
+    e = s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), nullptr, s.blk->cap, s.st);
   if (e == hipSuccess)
     e = launch_sys_ranges(s.B.offsets, s.B.items, (int)S, 1, s.P.n_items, s.dranges, s.st);
   if (e == hipSuccess)
@@ -379,68 +460,51 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   return true;
 }
 
-// Counts are known: refill if the estimate was short, else copy the lists.
-bool Server::on_counted(Slot& s) {
+// Counts are known: deliver, or first refill into a larger block if the
+// estimate was short (the first fill pass then wrote nothing).
+void Server::on_counted(Slot& s) {
   const int64_t total = s.P.host_bounds(s.pin)[1];
-  s.total = total;
-  hipError_t e = hipSuccess;
-  if (total > s.out_cap) {
-    (void)hipFree(s.edges);
-    (void)hipFree(s.shifts);
-    s.edges = s.shifts = nullptr;
-    const int64_t cap = total + total / 8 + 4096;
-    if (hipMalloc(&s.edges, 16 * (size_t)cap) != hipSuccess ||
-        hipMalloc(&s.shifts, 12 * (size_t)cap) != hipSuccess) {
-      s.out_cap = 0;
-      fail_slot(s, "out of device memory for the batch's edge lists");
-      return false;
-    }
-    s.out_cap = cap;
-    e = s.P.enqueue_fill(s.B, s.edges, s.shifts, nullptr, s.out_cap, s.st);
+  if (total <= s.blk->cap) {
+    deliver(s, total);
+    return;
+  }
+  {
     std::lock_guard<std::mutex> g(mu_);
+    pool_.push_back(s.blk);
     ++refills_;
   }
-  if (!grow_pinned(&s.hout, &s.hout_cap, 28 * (size_t)std::max<int64_t>(total, 1))) {
-    fail_slot(s, "out of pinned memory for the results");
-    return false;
+  s.blk = get_block(total + total / 8 + 4096);
+  if (!s.blk) {
+    fail_slot(s, "out of device memory for the batch's edge lists");
+    return;
   }
-  if (total > 0) {
-    const int64_t* ed = (const int64_t*)s.edges;
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(s.hout, ed, 16 * (size_t)total, hipMemcpyDeviceToHost, s.st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(s.hout + 16 * (size_t)total, s.shifts, 12 * (size_t)total,
-                         hipMemcpyDeviceToHost, s.st);
-  }
+  hipError_t e =
+      s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), nullptr, s.blk->cap, s.st);
   if (e == hipSuccess) e = hipEventRecord(s.ev, s.st);
   if (e != hipSuccess) {
     fail_slot(s, std::string("HIP error in the server: ") + hipGetErrorString(e));
-    return false;
+    return;
   }
-  s.state = 3;
-  return true;
+  s.state = 2;
 }
 
-void Server::deliver(Slot& s) {
-  const int64_t total = s.total;
-  const int64_t* ei = (const int64_t*)s.hout;
-  const int64_t* ej = ei + total;
-  const int32_t* sh = (const int32_t*)(s.hout + 16 * (size_t)total);
+void Server::deliver(Slot& s, int64_t total) {
   const double t = now_s();
+  std::lock_guard<std::mutex> g(mu_);
+  s.blk->total = total;
+  s.blk->refs = (int)s.reqs.size();
   for (size_t k = 0; k < s.reqs.size(); ++k) {
     SrvReq* r = s.reqs[k];
-    const int64_t a = s.hranges[2 * k], b = s.hranges[2 * k + 1];
-    r->ei.assign(ei + a, ei + b);
-    r->ej.assign(ej + a, ej + b);
-    r->sh.assign(sh + 3 * a, sh + 3 * b);
-  }
-  std::lock_guard<std::mutex> g(mu_);
-  for (SrvReq* r : s.reqs) {
+    r->blk = s.blk;
+    r->first = s.hranges[2 * k];
+    r->len = s.hranges[2 * k + 1] - r->first;
     r->t_done = t;
     r->state = 2;
     ++completed_;
   }
-  inflight_est_ -= s.cap_used;
+  s.blk = nullptr;
+  inflight_est_ -= s.est_used;
+  s.est_used = 0;
   s.reqs.clear();
   s.state = 0;
   done_cv_.notify_all();
@@ -448,57 +512,52 @@ void Server::deliver(Slot& s) {
 
 void Server::loop() {
   (void)hipSetDevice(eng_->device());
-  int batch_id = 0;
   while (true) {
     bool progress = false;
     // 1. advance slots whose device work finished
     for (Slot* s : slots_) {
-      if (s->state == 0) continue;
-      if (hipEventQuery(s->ev) != hipSuccess) continue;
+      if (s->state == 0 || hipEventQuery(s->ev) != hipSuccess) continue;
       progress = true;
       if (s->state == 1)
         on_counted(*s);
-      else if (s->state == 3)
-        deliver(*s);
+      else
+        deliver(*s, s->P.host_bounds(s->pin)[1]);
     }
     // 2. admit queued requests into free slots
     for (Slot* s : slots_) {
       if (s->state != 0) continue;
       std::vector<SrvReq*> batch;
+      int64_t est = 0;
       {
         std::lock_guard<std::mutex> g(mu_);
         if (queue_.empty()) break;
-        int64_t atoms = 0, est = 0;
+        int64_t atoms = 0;
         int busy = 0;
         for (Slot* o : slots_) busy += o->state != 0;
         while (!queue_.empty()) {
           SrvReq* r = queue_.front();
           const int64_t na = (int64_t)(r->pos.size() / 3);
-          const int64_t ce = r->est + r->est / 4 + 4096;
           const bool first = batch.empty();
           if (!first && (atoms + na > max_atoms_ || (int64_t)batch.size() >= max_systems_))
             break;
-          // edge-memory admission: a lone request is admitted when nothing
+          // edge-budget admission; a lone request is admitted when nothing
           // else is on the device, so oversized requests cannot starve
-          if (inflight_est_ + est + ce > budget_ && !(first && busy == 0 && inflight_est_ == 0))
-            break;
+          if (inflight_est_ + est + r->est > budget_ && !(first && busy == 0)) break;
           batch.push_back(r);
           atoms += na;
-          est += ce;
+          est += r->est;
           queue_.pop_front();
         }
         if (batch.empty()) break;
         inflight_est_ += est;
         for (SrvReq* r : batch) {
           r->state = 1;
-          r->batch = batch_id;
           r->batch_size = (int)batch.size();
-          r->slot = s->index;
         }
         ++batches_;
         max_batch_ = std::max<int64_t>(max_batch_, (int64_t)batch.size());
       }
-      s->batch = batch_id++;
+      s->est_used = est;
       if (launch(*s, batch)) progress = true;
       int busy = 0;
       for (Slot* o : slots_) busy += o->state != 0;
@@ -513,7 +572,7 @@ void Server::loop() {
     if (!progress) {
       if (any_busy)
         cv_.wait_for(g, std::chrono::microseconds(20));
-      else
+      else if (queue_.empty() && !stop_)
         cv_.wait_for(g, std::chrono::milliseconds(5));
     }
   }

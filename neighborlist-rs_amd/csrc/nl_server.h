@@ -13,6 +13,7 @@ namespace nl {
 
 class Engine;
 struct SrvReq;
+struct OutBlock;
 
 int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc);
 
@@ -20,12 +21,15 @@ class Server {
  public:
   Server(Engine* eng, int slots, int64_t max_atoms, int64_t max_systems, int64_t edge_budget);
   ~Server();
-  // returns the request id (> 0) or -1 with nl_last_error set
+  // returns the request id (> 0) or -1 with the thread's error message set
   int64_t submit(const double* pos, int64_t n, const double* h, const uint8_t* pbc,
                  double cutoff);
   int wait(int64_t id, double timeout_s);  // 0 done, 1 timed out, -1 failed
   int info(int64_t id, int64_t* n_edges, double* latency_s, int* batch_size);
-  int take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh);  // copies, then forgets
+  // device addresses of a completed request's lists (valid until release/take)
+  int device_view(int64_t id, const int64_t** ei, const int64_t** ej, const int32_t** sh);
+  int release(int64_t id);
+  int take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh);  // copies out, then releases
   void stats(int64_t* completed, int64_t* batches, int64_t* max_inflight, int64_t* max_batch,
              int64_t* refills);
 
@@ -33,9 +37,12 @@ class Server {
   struct Slot;
   void loop();
   bool launch(Slot& s, std::vector<SrvReq*>& batch);
-  bool on_counted(Slot& s);
-  void deliver(Slot& s);
+  void on_counted(Slot& s);
+  void deliver(Slot& s, int64_t total);
   void fail_slot(Slot& s, const std::string& msg);
+  OutBlock* get_block(int64_t cap);
+  SrvReq* find_done(int64_t id);  // mu_ held
+  void drop_locked(SrvReq* r);     // mu_ held
 
   Engine* eng_;
   int64_t max_atoms_, max_systems_, budget_;
@@ -45,9 +52,10 @@ class Server {
   std::condition_variable cv_, done_cv_;
   std::deque<SrvReq*> queue_;
   std::unordered_map<int64_t, SrvReq*> reqs_;
+  std::vector<OutBlock*> pool_;  // free output blocks
   bool stop_ = false;
-  int64_t next_id_ = 1, submitted_ = 0, completed_ = 0, batches_ = 0, max_inflight_ = 0,
-          max_batch_ = 0, inflight_est_ = 0, refills_ = 0;
+  int64_t next_id_ = 1, completed_ = 0, batches_ = 0, max_inflight_ = 0, max_batch_ = 0,
+          inflight_est_ = 0, refills_ = 0;
 };
 
 }  // namespace nl

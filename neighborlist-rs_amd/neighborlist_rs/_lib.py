@@ -84,6 +84,9 @@ _SIGS = {
     "nl_server_result_info": (C.c_int, [C.c_void_p, C.c_int64, i64p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int)]),
     "nl_server_take": (C.c_int, [C.c_void_p, C.c_int64, i64p, i64p, i32p]),
+    "nl_server_result_device": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "nl_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "nl_server_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p, i64p]),
     "nl_server_free": (None, [C.c_void_p]),
     "nl_cell_inverse": (C.c_int, [dp, dp]),

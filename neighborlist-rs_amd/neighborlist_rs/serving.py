@@ -89,6 +89,23 @@ class Server:
             raise RuntimeError(_lib.last_error())
         return {"edge_index": ei, "shift": sh}
 
+    def result_device(self, rid, timeout=None):
+        """Blocks until done; returns (edges, addr_i, addr_j, addr_shift): device
+        addresses of the request's lists (M int64, M int64, M x 3 int32), valid
+        until release(rid)."""
+        if not self.wait(rid, timeout):
+            raise TimeoutError(f"request {rid} not done after {timeout} s")
+        m, _, _ = self.info(rid)
+        pi, pj, ps = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        if lib.nl_server_result_device(self._p, int(rid), C.byref(pi), C.byref(pj),
+                                       C.byref(ps)):
+            raise RuntimeError(_lib.last_error())
+        return m, pi.value or 0, pj.value or 0, ps.value or 0
+
+    def release(self, rid):
+        if lib.nl_server_release(self._p, int(rid)):
+            raise RuntimeError(_lib.last_error())
+
     def stats(self):
         v = [C.c_int64() for _ in range(5)]
         lib.nl_server_stats(self._p, *[C.byref(x) for x in v])

@@ -115,3 +115,26 @@ def test_serving_errors(nr):
             srv.submit(nr.PyCell([[1, 0, 0], [0, 1, 0], [0, 0, 0]], [True] * 3), p, 2.0)
         r = srv.result(srv.submit(None, np.zeros((1, 3)), 2.0), 60.0)
         assert r["edge_index"].shape == (2, 0)
+
+
+def test_serving_device_results(nr, oracle):
+    import ctypes as C
+    from neighborlist_rs import _lib
+    reqs = _trace(seed=13, n=12, nmax=2500)
+    _print_case("serving device results", len(reqs))
+    with nr.Server(slots=2, max_atoms=6000) as srv:
+        ids = [srv.submit(_cell(nr, H), p, rc) for (_, p, H, rc) in reqs]
+        for k, rid in enumerate(ids):
+            _, p, H, rc = reqs[k]
+            m, pi, pj, ps = srv.result_device(rid, 60.0)
+            ei = np.empty((2, m), np.int64)
+            sh = np.empty((m, 3), np.int32)
+            if m:
+                for dst, src, nb in ((ei[0], pi, 8 * m), (ei[1], pj, 8 * m), (sh, ps, 12 * m)):
+                    assert _lib.lib.nl_memcpy_d2h(dst.ctypes.data_as(C.c_void_p),
+                                                  C.c_void_p(src), nb) == 0
+            srv.release(rid)
+            with pytest.raises(RuntimeError, match="not complete"):
+                srv.release(rid)
+            want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
+            assert_same_edges((ei[0], ei[1], sh), want, f"request {k}")
