@@ -31,6 +31,29 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 
 
+FILL_PROFILE = os.path.join(ROOT, "profiles", "k_pairs_fill_c4_ragged_batch.md")
+
+
+def profiled_traffic(edges):
+    """HBM bytes per fill launch measured by rocprofv3 --pmc on the same
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel
+    calibration), read from the committed profile summary; None when the
+    profile is missing or was taken on a different edge count."""
+    vals = {}
+    try:
+        with open(FILL_PROFILE) as f:
+            for line in f:
+                if ":" in line and line.split(":")[0].strip() in ("traffic_bytes_per_launch",
+                                                                  "edges_per_launch"):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = int(v.strip().strip("`"))
+    except OSError:
+        return None
+    if vals.get("edges_per_launch") != edges:
+        return None
+    return vals.get("traffic_bytes_per_launch")
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -306,8 +329,12 @@ def main():
     fill_ms = float(np.mean(fills))
     alg_bytes = 28.0 * edges + 24.0 * n
     achieved = alg_bytes / (fill_ms * 1e-3) / 1e9
+    traffic = profiled_traffic(edges)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "traffic_source": None if traffic is None else
+                "profiles/k_pairs_fill_c4_ragged_batch.md (rocprofv3 --pmc FETCH_SIZE, "
+                "WRITE_SIZE; copy-calibrated)",
                 "kernel": "k_pairs<fill>", "kernel_ms": round(fill_ms, 4),
                 "alg_bytes_per_launch": int(alg_bytes)}
 

@@ -219,3 +219,39 @@ def test_work_split_does_not_change_results(nr, oracle, group, parts):
             assert_same_edges(as_tuple(got[k]), want[k], f"G={group} P={parts} list {k}")
     finally:
         lib.nl_set_work_split(8, 2)
+
+
+@pytest.mark.parametrize("disjoint", [False, True])
+def test_twenty_cutoffs_vs_oracle(nr, oracle, disjoint):
+    """More lists than one pair-kernel launch holds (8): several launches per pass,
+    unsorted input order, duplicates (search.rs:354-360 stable sort)."""
+    from neighborlist_rs import workloads as W
+    rng = np.random.default_rng(20)
+    cut = [float(c) for c in rng.uniform(2.0, 9.0, size=18)] + [4.0, 4.0]
+    p, H, _ = W.c2_ethanol()
+    _print_case("twenty cutoffs single", disjoint)
+    want = oracle.search_single_multi(p, H, [1, 1, 1], cut, disjoint)
+    got = nr.build_neighborlists_multi(nr.PyCell(H.tolist()), p, cut, disjoint=disjoint)
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"single list {k} rc={cut[k]:.3f}")
+    p, sizes, cells, _ = W.c4_ragged_batch(n_systems=16, nmax=1100, seed=21)
+    batch, rows = W.batch_arrays(sizes, cells)
+    _print_case("twenty cutoffs batch", disjoint)
+    ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+    want = oracle.search_batch_multi(p, batch, ocells, cut, disjoint)
+    got = nr.build_neighborlists_batch_multi(p, batch, rows, cut, disjoint=disjoint)
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"batch list {k} rc={cut[k]:.3f}")
+
+
+def test_parallel_flag_does_not_change_results(nr, oracle):
+    """parallel=False selects the reference's serial search (search.rs:275-289);
+    the edge set is the same."""
+    from neighborlist_rs import workloads as W
+    p, H, _ = W.c2_ethanol()
+    _print_case("parallel flag")
+    cell = nr.PyCell(H.tolist())
+    a = as_tuple(nr.build_neighborlists(cell, p, 6.0, parallel=True))
+    b = as_tuple(nr.build_neighborlists(cell, p, 6.0, parallel=False))
+    assert_same_edges(a, b, "parallel vs serial")
+    assert_same_edges(b, oracle.search_single(p, H, [1, 1, 1], 6.0, parallel=False), "serial")
