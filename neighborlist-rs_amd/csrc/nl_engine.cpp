@@ -156,6 +156,7 @@ std::vector<size_t> Pipeline::buffer_sizes() const {
       4 * (size_t)scan_len,                                         // counts
       8 * (size_t)(scan_len + 1),                                   // offsets
       8 * (size_t)(K + 1),                                          // bounds
+      8 * (size_t)n_items,                                          // work list
   };
 }
 
@@ -178,6 +179,7 @@ void Pipeline::assign(Buffers& b, const std::vector<void*>& p) {
   b.counts = (uint32_t*)p[15];
   b.offsets = (int64_t*)p[16];
   b.bounds = (int64_t*)p[17];
+  b.list = (int64_t*)p[18];
 }
 
 size_t Pipeline::pinned_bytes() const { return off_bounds() + 8 * (size_t)(K + 1) + 64; }
@@ -223,6 +225,8 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.band = b.band;
   A.counts = b.counts;
   A.offsets = b.offsets;
+  A.list = b.list;
+  A.n_real = b.zero + 2 * P.stiles + 256 * P.passes + 256 * P.rtiles * P.passes + 6;
   A.bounds = b.bounds;
   A.n = P.n_items;
   A.n_sys = P.S;
@@ -253,6 +257,10 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   }
   NL_TRY(launch_finalize(kin, vin, b.pos, b.sys, b.start, S, n, bin_total, b.cell_start,
                          b.spos64, b.spos32, b.sidx, st));
+  // work list of the items that have work; counts of the others stay zero
+  NL_TRY(launch_fill_u32(b.counts, scan_len, 0, st));
+  NL_TRY(launch_items(kin, b.cell_start, b.sys, b.items, S, n_items, tune.group, tune.parts,
+                      b.list, tickets + 6, st));
   PairArgs A = pair_args(*this, b);
   for (int q0 = 0; q0 < K; q0 += kGroup) {
     A.q_lo = q0;

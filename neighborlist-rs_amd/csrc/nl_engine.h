@@ -77,8 +77,9 @@ struct Buffers {
   uint32_t* counts = nullptr;
   int64_t* offsets = nullptr;
   int64_t* bounds = nullptr;
+  int64_t* list = nullptr;  // work items that have work (any order)
 };
-constexpr int kNumBuffers = 18;
+constexpr int kNumBuffers = 19;
 
 // Host-prepared launch sequence for one request shape.
 class Pipeline {

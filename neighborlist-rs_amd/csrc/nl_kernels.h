@@ -21,7 +21,9 @@ struct PairArgs {
   const int4* sidx;          // sorted wrap shifts s_a, w = local index
   const double* rc2;         // k squared cutoffs, ascending ([n_sys][k] if per_sys_rc)
   const float2* band;        // [n_sys][k] FP32 accept/reject thresholds
-  uint32_t* counts;          // [k][n] (count pass)
+  uint32_t* counts;          // [k][n] (count pass; zero for items without work)
+  const int64_t* list;       // items with work, one per wave (any order)
+  const uint32_t* n_real;    // length of list (device)
   const int64_t* offsets;    // [k][n] + 1 (fill pass)
   const int64_t* bounds;     // [k + 1] first edge of each list (sorted-cutoff order)
   int64_t* edges;            // list q: sources at base_q + e, targets at base_q + M_q + e
@@ -48,6 +50,9 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
                            const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,
                            int64_t total_bins, int32_t* cell_start, Pos64* spos64,
                            float4* spos32, int4* sidx, hipStream_t st);
+hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const SysDesc* sys,
+                        const int64_t* item_base, int n_sys, int64_t n_items, int group,
+                        int parts, int64_t* list, uint32_t* count, hipStream_t st);
 hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
                        int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st);

@@ -18,6 +18,8 @@
 // decision is the reference's, term for term.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "nl_kernels.h"
 
 namespace nl {
@@ -433,8 +435,6 @@ __global__ __launch_bounds__(256) void k_finalize(
 namespace {
 
 struct WaveLds {
-  int32_t incl[64];   // inclusive candidate prefix per stencil segment
-  int32_t start[64];  // first sorted slot of the segment's bin
   int32_t bs[3][64];  // bin-image shift of the segment
   double off[3][64];  // H bs in FP64, reference order
   float off32[3][64]; // H bs in FP32 (filter)
@@ -491,12 +491,22 @@ __device__ __forceinline__ uint32_t members32(float e2, const Cuts<NQ>& C, bool*
   return m;
 }
 
+// Per-list accumulators: lane-private counts in the count pass (summed
+// over the wave when the work item ends), wave-uniform write cursors in
+// the fill pass.
+template <bool FILL>
+using Acc = typename std::conditional<FILL, int64_t, uint32_t>::type;
+
 // Emit or count one step of up to 64 candidates; `mem` holds the lane's
-// list bits.  `cur` holds the wave's running counts (count pass) or write
-// cursors (fill pass).
+// list bits (zero for lanes without a pair).
 template <bool FILL, int NQ>
 __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
-                                 int64_t gj, int sx, int sy, int sz, double dd, int64_t* cur) {
+                                 int64_t gj, int sx, int sy, int sz, double dd, Acc<FILL>* cur) {
+  if (!FILL) {
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) cur[g] += (mem >> g) & 1u;
+    return;
+  }
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
     if (!C.has(g)) break;
@@ -519,7 +529,7 @@ __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, 
 
 template <bool FILL, int NQ>
 __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
-                                int mode, int64_t* cur) {
+                                int mode, Acc<FILL>* cur) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int n = d->n_atoms;
@@ -566,7 +576,7 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
       d2 = norm2(dx, dy, dz);
     }
     const uint32_t mem = valid ? members64(d2, C) : 0u;
-    if (__ballot(mem != 0) == 0) continue;
+    if (FILL && __ballot(mem != 0) == 0) continue;
     const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
     emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
   }
@@ -597,9 +607,9 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 // orientation (lower original index first, S = s_j - s_i + bs from it,
 // search.rs:695-697); a self image (same atom, bs != 0) is written for both
 // S and -S, as the reference emits both (search.rs:549).
-template <bool FILL, int NQ>
+template <bool FILL, int NQ, bool F32>
 __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
-                          int part, WaveLds& L, int64_t* cur) {
+                          int part, WaveLds& L, Acc<FILL>* cur) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int nbx = d->nb[0], nby = d->nb[1], nbz = d->nb[2];
@@ -613,8 +623,6 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
   const double h0 = d->h[0], h1 = d->h[1], h2 = d->h[2], h3 = d->h[3], h4 = d->h[4],
                h5 = d->h[5], h6 = d->h[6], h7 = d->h[7], h8 = d->h[8];
-  const float g0 = d->h32[0], g1 = d->h32[1], g2 = d->h32[2], g3 = d->h32[3], g4 = d->h32[4],
-              g5 = d->h32[5], g6 = d->h32[6], g7 = d->h32[7], g8 = d->h32[8];
   const int wx = 2 * nsx + 1, wy = 2 * nsy + 1, wz = 2 * nsz + 1;
   const int T = wx * wy * wz;
   const int tc = (nsx * wy + nsy) * wz + nsz;  // the (0,0,0) offset
@@ -622,13 +630,6 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   const int slice = (T - tc + A.parts - 1) / A.parts;
   const int t_lo = tc + part * slice;
   const int t_hi = min(T, t_lo + slice);
-  const bool fp32 = A.use_fp32;
-  // beyond the group's largest band: no list (unrolled so that C stays in
-  // registers; a runtime index would put it in scratch memory)
-  float rej32 = -INFINITY;
-#pragma unroll
-  for (int g = 0; g < NQ; ++g)
-    if (C.has(g)) rej32 = fmaxf(rej32, C.hi[g]);
   for (int hb = hs; hb < he; hb += 64) {
     const int hn = min(64, he - hb);
     Pos64 hp = {0.0, 0.0, 0.0, 0.0};
@@ -637,7 +638,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
     if (lane < hn) {
       hp = A.spos64[hb + lane];
       hsi = A.sidx[hb + lane];
-      if (fp32) hp32 = A.spos32[hb + lane];
+      if (F32) hp32 = A.spos32[hb + lane];
     }
     for (int t0 = t_lo; t0 < t_hi; t0 += 64) {
       const int t = t0 + lane;
@@ -663,15 +664,18 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
         oz = (h6 * (double)bsx + h7 * (double)bsy) + h8 * (double)bsz;
       }
       const int incl = wave_incl_scan(len);
-      L.incl[lane] = incl;
-      L.start[lane] = st;
+      // candidate c of this segment list sits at sorted slot base[seg] + c
+      const int base = st - (incl - len);
       L.bs[0][lane] = bsx;
       L.bs[1][lane] = bsy;
       L.bs[2][lane] = bsz;
       L.off[0][lane] = ox;
       L.off[1][lane] = oy;
       L.off[2][lane] = oz;
-      if (fp32) {
+      if (F32) {
+        const float g0 = d->h32[0], g1 = d->h32[1], g2 = d->h32[2], g3 = d->h32[3],
+                    g4 = d->h32[4], g5 = d->h32[5], g6 = d->h32[6], g7 = d->h32[7],
+                    g8 = d->h32[8];
         L.off32[0][lane] = (g0 * (float)bsx + g1 * (float)bsy) + g2 * (float)bsz;
         L.off32[1][lane] = (g3 * (float)bsx + g4 * (float)bsy) + g5 * (float)bsz;
         L.off32[2][lane] = (g6 * (float)bsx + g7 * (float)bsy) + g8 * (float)bsz;
@@ -680,98 +684,167 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int total = __shfl(incl, 63, 64);
-      int seg0 = 0;  // first segment that still holds candidates at c0 (uniform)
-      for (int c0 = 0; c0 < total; c0 += 64) {
-        while (L.incl[seg0] <= c0) ++seg0;
-        const int idx = c0 + lane;
-        const bool valid = idx < total;
-        int sg = seg0;
-        if (valid)
-          while (L.incl[sg] <= idx) ++sg;
-        const int excl = sg > 0 ? L.incl[sg - 1] : 0;
-        const int j = L.start[sg] + (idx - excl);
-        const bool home_seg = (t0 == tc) && (sg == 0);
-        const double cox = L.off[0][sg], coy = L.off[1][sg], coz = L.off[2][sg];
-        const int sbx = L.bs[0][sg], sby = L.bs[1][sg], sbz = L.bs[2][sg];
-        Pos64 c = {0.0, 0.0, 0.0, 0.0};
-        float4 c32 = make_float4(0.f, 0.f, 0.f, 0.f);
-        float fox = 0.f, foy = 0.f, foz = 0.f;
-        int4 cj = make_int4(0, 0, 0, 0);
-        int oj = 0;
-        if (valid) {
-          if (fp32) {
-            c32 = A.spos32[j];
-            oj = __float_as_int(c32.w);
-            fox = L.off32[0][sg];
-            foy = L.off32[1][sg];
-            foz = L.off32[2][sg];
-          } else {
-            c = A.spos64[j];
-            oj = (int)c.w;
+      if (total > 0) {
+        // Candidates stream through the lanes 64 at a time.  The loads of
+        // the next chunk are issued before the current one is tested, so
+        // their latency overlaps the home-atom loop.
+        int sg = 0, j = 0;
+        bool valid = false;
+        float4 n32 = make_float4(0.f, 0.f, 0.f, 0.f);
+        Pos64 n64 = {0.0, 0.0, 0.0, 0.0};
+        int4 nij = make_int4(0, 0, 0, 0);
+        // This lane's candidate of the chunk at c0: its segment is the number
+        // of segments whose inclusive count is <= c0 + lane.  Lane s holds
+        // segment s's count, so the segments ending before the chunk are
+        // one ballot, and the few ending inside it are walked in order.
+        auto locate = [&](int c0) {
+          const int idx = c0 + lane;
+          valid = idx < total;
+          sg = (int)__popcll(__ballot(incl <= c0));
+          uint64_t m = __ballot(incl > c0 && incl <= c0 + 63);
+          while (m) {
+            const int e = (int)__builtin_ctzll(m);
+            sg += lane >= __builtin_amdgcn_readlane(incl, e) - c0 ? 1 : 0;
+            m &= m - 1;
           }
-          if (FILL) cj = A.sidx[j];
-        }
-        for (int i = 0; i < hn; ++i) {
-          const int oi = __builtin_amdgcn_readlane(hsi.w, i);
-          const bool cand = valid && (!home_seg || j > hb + i);
-          uint32_t mem = 0;
-          double d2 = INFINITY;
-          if (fp32) {
-            const float xi = readlane_f(hp32.x, i), yi = readlane_f(hp32.y, i),
-                        zi = readlane_f(hp32.z, i);
-            const float dx = (c32.x - xi) + fox;
-            const float dy = (c32.y - yi) + foy;
-            const float dz = (c32.z - zi) + foz;
-            const float e2 = (dx * dx + dy * dy) + dz * dz;
-            bool unsure = false;
-            const bool near = cand && e2 <= rej32;
-            if (near) mem = members32(e2, C, &unsure);
-            const bool need64 = near && (unsure || A.dist != nullptr);
-            if (__ballot(need64)) {
-              const double xi64 = readlane_d(hp.x, i), yi64 = readlane_d(hp.y, i),
-                           zi64 = readlane_d(hp.z, i);
-              if (need64) {
-                // FP64 recheck: the reference expression, exactly
-                const Pos64 c64 = A.spos64[j];
-                d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy, (c64.z - zi64) + coz);
-                mem = members64(d2, C);
+          sg = min(sg, 63);  // lanes past the last candidate
+          j = __shfl(base, sg, 64) + idx;
+        };
+        auto fetch = [&]() {
+          if (valid) {
+            if (F32)
+              n32 = A.spos32[j];
+            else
+              n64 = A.spos64[j];
+            if (FILL) nij = A.sidx[j];
+          }
+        };
+        locate(0);
+        fetch();
+        for (int c0 = 0; c0 < total; c0 += 64) {
+          const int csg = sg, cjs = j;
+          const bool cvalid = valid;
+          const float4 c32 = n32;
+          const Pos64 c = n64;
+          const int4 cj = nij;
+          if (c0 + 64 < total) {
+            locate(c0 + 64);
+            fetch();
+          }
+          const bool home_seg = (t0 == tc) && (csg == 0);
+          const double cox = L.off[0][csg], coy = L.off[1][csg], coz = L.off[2][csg];
+          const int sbx = L.bs[0][csg], sby = L.bs[1][csg], sbz = L.bs[2][csg];
+          // FP32 filter coordinates: candidate plus image offset, once per
+          // candidate (the band covers this order of rounding as well)
+          float px = 0.f, py = 0.f, pz = 0.f;
+          if (F32) {
+            px = c32.x + L.off32[0][csg];
+            py = c32.y + L.off32[1][csg];
+            pz = c32.z + L.off32[2][csg];
+          }
+          const int oj = F32 ? __float_as_int(c32.w) : (int)c.w;
+#pragma unroll 1
+          for (int i = 0; i < hn; ++i) {
+            const int oi = __builtin_amdgcn_readlane(hsi.w, i);
+            const bool cand = cvalid && (!home_seg || cjs > hb + i);
+            if (!FILL && !C.disjoint) {
+              // Count pass, cumulative lists: lane counters, no ballots.  A
+              // self image (same atom, other image) counts twice (both S and
+              // -S are emitted).  Lists past q_cnt have -inf thresholds.
+              const uint32_t w1 = cand ? 1u + (uint32_t)(oj == oi) : 0u;
+              if (F32) {
+                const float xi = readlane_f(hp32.x, i), yi = readlane_f(hp32.y, i),
+                            zi = readlane_f(hp32.z, i);
+                const float dx = px - xi, dy = py - yi, dz = pz - zi;
+                const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                bool unsure = false;
+#pragma unroll
+                for (int g = 0; g < NQ; ++g) {
+                  const bool acc = e2 < C.lo[g];
+                  cur[g] += acc ? w1 : 0u;
+                  unsure |= !acc && e2 <= C.hi[g];
+                }
+                unsure = unsure && cand;
+                if (__ballot(unsure)) {
+                  const double xi64 = readlane_d(hp.x, i), yi64 = readlane_d(hp.y, i),
+                               zi64 = readlane_d(hp.z, i);
+                  if (unsure) {
+                    // FP64 recheck: the reference expression, exactly
+                    const Pos64 c64 = A.spos64[cjs];
+                    const double d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy,
+                                            (c64.z - zi64) + coz);
+#pragma unroll
+                    for (int g = 0; g < NQ; ++g)
+                      cur[g] += (!(e2 < C.lo[g]) && d2 < C.r[g]) ? w1 : 0u;
+                  }
+                }
+              } else {
+                const double xi = readlane_d(hp.x, i), yi = readlane_d(hp.y, i),
+                             zi = readlane_d(hp.z, i);
+                // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
+                const double d2 = norm2((c.x - xi) + cox, (c.y - yi) + coy, (c.z - zi) + coz);
+#pragma unroll
+                for (int g = 0; g < NQ; ++g) cur[g] += d2 < C.r[g] ? w1 : 0u;
+              }
+              continue;
+            }
+            uint32_t mem = 0;
+            double d2 = INFINITY;
+            if (F32) {
+              const float xi = readlane_f(hp32.x, i), yi = readlane_f(hp32.y, i),
+                          zi = readlane_f(hp32.z, i);
+              const float dx = px - xi, dy = py - yi, dz = pz - zi;
+              const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              bool unsure = false;
+              mem = members32(e2, C, &unsure);
+              mem = cand ? mem : 0u;
+              const bool need64 = cand && (unsure || (A.dist != nullptr && mem != 0));
+              if (__ballot(need64)) {
+                const double xi64 = readlane_d(hp.x, i), yi64 = readlane_d(hp.y, i),
+                             zi64 = readlane_d(hp.z, i);
+                if (need64) {
+                  // FP64 recheck: the reference expression, exactly
+                  const Pos64 c64 = A.spos64[cjs];
+                  d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy, (c64.z - zi64) + coz);
+                  mem = members64(d2, C);
+                }
+              }
+            } else {
+              const double xi = readlane_d(hp.x, i), yi = readlane_d(hp.y, i),
+                           zi = readlane_d(hp.z, i);
+              // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
+              d2 = norm2((c.x - xi) + cox, (c.y - yi) + coy, (c.z - zi) + coz);
+              if (cand) mem = members64(d2, C);
+            }
+            if (FILL && __ballot(mem != 0) == 0) continue;
+            const bool self_img = (mem != 0) && (oj == oi);
+            int sx = 0, sy = 0, sz = 0;
+            int64_t gi = 0, gj = 0;
+            if (FILL) {
+              const int six = __builtin_amdgcn_readlane(hsi.x, i);
+              const int siy = __builtin_amdgcn_readlane(hsi.y, i);
+              const int siz = __builtin_amdgcn_readlane(hsi.z, i);
+              if (mem) {
+                sx = cj.x - six + sbx;
+                sy = cj.y - siy + sby;
+                sz = cj.z - siz + sbz;
+                int lo_i = oi, hi_i = oj;
+                if (oj < oi) {
+                  lo_i = oj;
+                  hi_i = oi;
+                  sx = -sx;
+                  sy = -sy;
+                  sz = -sz;
+                }
+                gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
+                gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
               }
             }
-          } else {
-            const double xi = readlane_d(hp.x, i), yi = readlane_d(hp.y, i),
-                         zi = readlane_d(hp.z, i);
-            // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-            d2 = norm2((c.x - xi) + cox, (c.y - yi) + coy, (c.z - zi) + coz);
-            if (cand) mem = members64(d2, C);
+            const double dd = A.dist ? sqrt(d2) : 0.0;
+            emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, dd, cur);
+            if (!FILL || __ballot(self_img))
+              emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, dd, cur);
           }
-          if (__ballot(mem != 0) == 0) continue;
-          const bool self_img = (mem != 0) && (oj == oi);
-          int sx = 0, sy = 0, sz = 0;
-          int64_t gi = 0, gj = 0;
-          if (FILL) {
-            const int six = __builtin_amdgcn_readlane(hsi.x, i);
-            const int siy = __builtin_amdgcn_readlane(hsi.y, i);
-            const int siz = __builtin_amdgcn_readlane(hsi.z, i);
-            if (mem) {
-              sx = cj.x - six + sbx;
-              sy = cj.y - siy + sby;
-              sz = cj.z - siz + sbz;
-              int lo_i = oi, hi_i = oj;
-              if (oj < oi) {
-                lo_i = oj;
-                hi_i = oi;
-                sx = -sx;
-                sy = -sy;
-                sz = -sz;
-              }
-              gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
-              gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
-            }
-          }
-          const double dd = A.dist ? sqrt(d2) : 0.0;
-          emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, dd, cur);
-          if (__ballot(self_img))
-            emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, dd, cur);
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -787,15 +860,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
 // takes the cell-list path; slots that do not start a group are empty).
 // Edges of one system are thus contiguous and systems come out in input
 // order, as batch.rs:135-144 concatenates them.
-template <bool FILL, int NQ>
-__global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
-  __shared__ WaveLds s_lds[kWaves];
-  const int w = threadIdx.x >> 6;
-  const int64_t u = (int64_t)blockIdx.x * kWaves + w;
-  if (u >= A.n) return;
-  // fill pass launched before the host saw the counts: write nothing when
-  // the lists do not fit; the host re-runs the fill with exact buffers
-  if (FILL && A.bounds[A.k] > A.capacity) return;
+template <bool FILL, int NQ, bool F32>
+__device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds& L) {
   const int s = uniform_i32(find_sys(A.item_base, A.n_sys, u));
   const SysDesc* d = A.sys + s;
   const int mode = uniform_i32(d->mode);
@@ -810,7 +876,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   const float2 bp = A.q_lo > 0 ? band[A.q_lo - 1] : make_float2(-INFINITY, -INFINITY);
   C.prev_lo = bp.x;
   C.prev_hi = bp.y;
-  int64_t cur[NQ];
+  Acc<FILL> cur[NQ];
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
     const bool on = g < A.q_cnt;
@@ -829,13 +895,68 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   } else if (mode == kCellList) {
     const int64_t r = local - n_atoms;
     const int64_t p = d->atom_start + r / A.parts;
-    unit_cell<FILL, NQ>(A, C, d, p, (int)(r % A.parts), s_lds[w], cur);
+    unit_cell<FILL, NQ, F32>(A, C, d, p, (int)(r % A.parts), L, cur);
   }
-  if (!FILL && lane_id() == 0) {
+  if (!FILL) {
 #pragma unroll
-    for (int g = 0; g < NQ; ++g)
-      if (g < A.q_cnt) A.counts[(int64_t)(A.q_lo + g) * A.n + u] = (uint32_t)cur[g];
+    for (int g = 0; g < NQ; ++g) {
+      uint32_t t = cur[g];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      if (g < A.q_cnt && lane_id() == 0) A.counts[(int64_t)(A.q_lo + g) * A.n + u] = t;
+    }
   }
+}
+
+// Work list: the items that have work (rows of brute-force systems, home
+// group starts of cell-list systems), appended in any order with one atomic
+// per wave.  The pair kernels map wave w to list[w], so blocks are full of
+// working waves (most item slots of a cell-list system start no group).
+// Which wave takes an item never changes its output: offsets are per item.
+__global__ __launch_bounds__(256) void k_items(const uint32_t* __restrict__ keys,
+                                               const int32_t* __restrict__ cell_start,
+                                               const SysDesc* __restrict__ sys,
+                                               const int64_t* __restrict__ item_base, int n_sys,
+                                               int64_t n_items, int group, int parts,
+                                               int64_t* __restrict__ list,
+                                               uint32_t* __restrict__ count) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool real = false;
+  if (u < n_items) {
+    const int s = find_sys(item_base, n_sys, u);
+    const SysDesc* d = sys + s;
+    const int64_t local = u - item_base[s];
+    const int n = d->n_atoms;
+    if (local < n) {
+      real = d->mode != kCellList;
+    } else if (d->mode == kCellList) {
+      const int64_t p = d->atom_start + (local - n) / parts;
+      const int bin_lo = cell_start[keys[p]];
+      real = ((int)p - bin_lo) % group == 0;
+    }
+  }
+  const uint64_t m = __ballot(real);
+  if (m == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (real) list[base + lanes_below(m)] = u;
+}
+
+// One wave per listed work item (a loop over several items per wave keeps
+// the per-item state live across iterations and costs about 40 VGPRs).
+// F32: FP32 filter with FP64 recheck in the band (cell-list items).
+template <bool FILL, int NQ, bool F32>
+__global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
+  __shared__ WaveLds s_lds[kWaves];
+  const int w = threadIdx.x >> 6;
+  const int64_t k = (int64_t)blockIdx.x * kWaves + w;
+  if (k >= (int64_t)*A.n_real) return;
+  const int64_t u = A.list[k];
+  // fill pass launched before the host saw the counts: write nothing when
+  // the lists do not fit; the host re-runs the fill with exact buffers
+  if (FILL && A.bounds[A.k] > A.capacity) return;
+  pair_item<FILL, NQ, F32>(A, u, s_lds[w]);
 }
 
 // ---------------------------------------------------------------------------
@@ -974,10 +1095,26 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
 template <int NQ>
 static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
   const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
-  if (fill)
-    hipLaunchKernelGGL((k_pairs<true, NQ>), g, b, 0, st, a);
-  else
-    hipLaunchKernelGGL((k_pairs<false, NQ>), g, b, 0, st, a);
+  if (a.use_fp32) {
+    if (fill)
+      hipLaunchKernelGGL((k_pairs<true, NQ, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_pairs<false, NQ, true>), g, b, 0, st, a);
+  } else {
+    if (fill)
+      hipLaunchKernelGGL((k_pairs<true, NQ, false>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_pairs<false, NQ, false>), g, b, 0, st, a);
+  }
+}
+
+hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const SysDesc* sys,
+                        const int64_t* item_base, int n_sys, int64_t n_items, int group,
+                        int parts, int64_t* list, uint32_t* count, hipStream_t st) {
+  if (n_items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_items, dim3(grid_for(n_items, 256)), dim3(256), 0, st, keys, cell_start,
+                     sys, item_base, n_sys, n_items, group, parts, list, count);
+  return hipGetLastError();
 }
 
 hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st) {
