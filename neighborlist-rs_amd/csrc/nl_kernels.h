@@ -23,7 +23,7 @@ struct PairArgs {
   const float2* band;        // [n_sys][k] FP32 accept/reject thresholds
   uint32_t* counts;          // [k][n] (count pass; zero for items without work)
   const int64_t* list;       // items with work, one per wave (any order)
-  const uint32_t* n_real;    // length of list (device)
+  const uint32_t* n_real;    // [2] list lengths from the front and from the back (device)
   const int64_t* offsets;    // [k][n] + 1 (fill pass)
   const int64_t* bounds;     // [k + 1] first edge of each list (sorted-cutoff order)
   int64_t* edges;            // list q: sources at base_q + e, targets at base_q + M_q + e
