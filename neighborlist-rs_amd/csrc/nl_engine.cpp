@@ -142,10 +142,8 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
 std::vector<size_t> Pipeline::buffer_sizes() const {
   return {
       pos_device ? 0 : sizeof(double) * 3 * (size_t)n,              // pos
-      sizeof(SysDesc) * S,                                          // sys
-      8 * (size_t)(S + 1),                                          // start
-      8 * (size_t)(S + 1),                                          // items
-      8 * rc2.size(),                                               // rc2
+      off_bounds(),            // sys | start | items | rc2: one upload
+      0, 0, 0,                 // (start, items, rc2 live in the block above)
       sizeof(float2) * (size_t)S * K,                               // band
       4 * (size_t)n, 4 * (size_t)n, 4 * (size_t)n, 4 * (size_t)n,  // keys/vals A/B
       4 * (size_t)z_words,                                          // zero
@@ -160,12 +158,12 @@ std::vector<size_t> Pipeline::buffer_sizes() const {
   };
 }
 
-void Pipeline::assign(Buffers& b, const std::vector<void*>& p) {
+void Pipeline::assign(Buffers& b, const std::vector<void*>& p) const {
   b.pos = (double*)p[0];
   b.sys = (SysDesc*)p[1];
-  b.start = (int64_t*)p[2];
-  b.items = (int64_t*)p[3];
-  b.rc2 = (double*)p[4];
+  b.start = (int64_t*)((char*)p[1] + off_start());
+  b.items = (int64_t*)((char*)p[1] + off_items());
+  b.rc2 = (double*)((char*)p[1] + off_rc2());
   b.band = (float2*)p[5];
   b.keysA = (uint32_t*)p[6];
   b.valsA = (uint32_t*)p[7];
@@ -202,10 +200,8 @@ int Pipeline::rank_of(int list) const {
 hipError_t Pipeline::enqueue_upload(const Buffers& b, const char* pin, hipStream_t st) const {
   if (!pos_device)
     NL_TRY(hipMemcpyAsync(b.pos, host_pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
-  NL_TRY(hipMemcpyAsync(b.sys, pin, sizeof(SysDesc) * S, hipMemcpyHostToDevice, st));
-  NL_TRY(hipMemcpyAsync(b.start, pin + off_start(), 8 * (S + 1), hipMemcpyHostToDevice, st));
-  NL_TRY(hipMemcpyAsync(b.items, pin + off_items(), 8 * (S + 1), hipMemcpyHostToDevice, st));
-  NL_TRY(hipMemcpyAsync(b.rc2, pin + off_rc2(), 8 * rc2.size(), hipMemcpyHostToDevice, st));
+  // descriptors, system starts, item bases and cutoffs: one contiguous copy
+  NL_TRY(hipMemcpyAsync(b.sys, pin, off_bounds(), hipMemcpyHostToDevice, st));
   return hipSuccess;
 }
 
@@ -245,7 +241,8 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   uint32_t* hist = b.zero + 2 * stiles;
   uint32_t* rstatus = hist + 256 * passes;
   uint32_t* tickets = rstatus + 256 * rtiles * passes;
-  NL_TRY(launch_fill_u32(b.zero, z_words, 0, st));
+  // zero block and counts (items without work keep zero counts)
+  NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st));
   NL_TRY(launch_plan(b.pos, b.sys, b.band, b.rc2, prm, st));
   NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, hist, passes, st));
   uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
@@ -257,8 +254,7 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   }
   NL_TRY(launch_finalize(kin, vin, b.pos, b.sys, b.start, S, n, bin_total, b.cell_start,
                          b.spos64, b.spos32, b.sidx, st));
-  // work list of the items that have work; counts of the others stay zero
-  NL_TRY(launch_fill_u32(b.counts, scan_len, 0, st));
+  // work list of the items that have work
   NL_TRY(launch_items(kin, b.cell_start, b.sys, b.items, S, n_items, tune.group, tune.parts,
                       b.list, tickets + 6, st));
   PairArgs A = pair_args(*this, b);
@@ -434,7 +430,7 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
     }
   }
   Buffers B;
-  Pipeline::assign(B, ptr);
+  P.assign(B, ptr);
   char* pin = (char*)pinned(P.pinned_bytes());
   if (!pin) {
     set_error("out of pinned host memory");
@@ -553,23 +549,30 @@ Tuning Engine::tuning_for(const Request& req) {
   Request probe = req;
   probe.want_ranges = false;
   probe.want_dist = false;
+  // device time of the search stages (hipEvents around the launches), best
+  // of three runs; a candidate must beat the incumbent by 2% to replace it,
+  // so timing noise does not walk the choice away from the default
   auto time_one = [&](const Tuning& x) -> double {
     double best = 1e30;
-    for (int rep = 0; rep < 2; ++rep) {
+    const bool was = timing_;
+    timing_ = true;
+    for (int rep = 0; rep < 3; ++rep) {
       Result* r = nullptr;
-      auto t0 = std::chrono::steady_clock::now();
-      if (run_with(probe, x, &r) != 0) return 1e30;
-      auto t1 = std::chrono::steady_clock::now();
+      if (run_with(probe, x, &r) != 0) {
+        timing_ = was;
+        return 1e30;
+      }
       delete r;
-      best = std::min(best, std::chrono::duration<double>(t1 - t0).count());
+      best = std::min(best, (double)times_.total);
     }
+    timing_ = was;
     return best;
   };
   Tuning best = t;
   double tb = time_one(best);
   auto consider = [&](Tuning x) {
     const double tx = time_one(x);
-    if (tx < tb) {
+    if (tx < 0.98 * tb) {
       tb = tx;
       best = x;
     }
@@ -705,7 +708,7 @@ int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
     }
     owned_.push_back(ptr[i]);
   }
-  Pipeline::assign(buf_, ptr);
+  pipe_.assign(buf_, ptr);
   if (hipHostMalloc((void**)&pinned_, pipe_.pinned_bytes(), hipHostMallocDefault) !=
       hipSuccess) {
     pinned_ = nullptr;

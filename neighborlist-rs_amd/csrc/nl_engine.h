@@ -89,7 +89,7 @@ class Pipeline {
 
   // sizes of each device buffer (bytes), in Buffers order (pos first)
   std::vector<size_t> buffer_sizes() const;
-  static void assign(Buffers& b, const std::vector<void*>& ptrs);
+  void assign(Buffers& b, const std::vector<void*>& ptrs) const;
   size_t pinned_bytes() const;
   void stage_host(char* pinned) const;  // descriptors etc. into pinned memory
 
@@ -114,7 +114,7 @@ class Pipeline {
   std::vector<SysDesc> desc;
   std::vector<int64_t> starts, item_base;
 
- private:
+  // layout of the descriptor block (pinned staging and device alike)
   size_t off_start() const { return sizeof(SysDesc) * S; }
   size_t off_items() const { return off_start() + 8 * (size_t)(S + 1); }
   size_t off_rc2() const { return off_items() + 8 * (size_t)(S + 1); }

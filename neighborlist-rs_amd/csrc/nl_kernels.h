@@ -38,6 +38,7 @@ struct PairArgs {
 };
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st);
 hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
                        const Params& p, hipStream_t st);
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,

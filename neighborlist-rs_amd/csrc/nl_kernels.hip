@@ -102,6 +102,19 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// Clears two word ranges in one launch (a kernel, not a memset node, so the
+// launch sequence stays safe to capture in a graph).
+__global__ void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restrict__ b,
+                        int64_t nb) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
+    if (i < na)
+      a[i] = 0u;
+    else
+      b[i - na] = 0u;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Planning: one block per system.  Restates single.rs:19-64 (auto-box,
 // Cell::new, perpendicular widths, mic_safe, brute-force dispatch) and picks
@@ -1086,6 +1099,14 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st) {
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st) {
+  if (na + nb <= 0) return hipSuccess;
+  int64_t g = (na + nb + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_zero2, dim3((unsigned)g), dim3(256), 0, st, a, na, b, nb);
   return hipGetLastError();
 }
 

@@ -418,7 +418,7 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
       fail_slot(s, "out of device memory in the server");
       return false;
     }
-  Pipeline::assign(s.B, s.bufs);
+  s.P.assign(s.B, s.bufs);
   const size_t S = batch.size();
   if (!grow_pinned((void**)&s.pin, &s.pin_cap, s.P.pinned_bytes()) ||
       !grow_pinned((void**)&s.hpos, &s.hpos_cap, sizeof(double) * 3 * (size_t)n) ||
