@@ -226,6 +226,57 @@ def serve_trace(L, n_req, slots, load):
     }
 
 
+def precision_report(L, pos, H, rc):
+    """FP64 |r_ij| emitted with the edges (cell-list path) against a float64
+    recomputation from the raw positions, |p_j + H S - p_i|, for the FP32
+    filter on and off (the edge sets must be identical)."""
+    lib = L.lib
+    out = {}
+    sets = []
+    old = lib.nl_get_fp32_filter()
+    old_tune = lib.nl_get_autotune()
+    lib.nl_set_autotune(0)  # the filter setting must reach the kernels
+    try:
+        for f32 in (0, 1):
+            lib.nl_set_fp32_filter(f32)
+            sz = np.array([pos.shape[0]], np.int64)
+            H9 = np.ascontiguousarray(np.asarray(H).reshape(9))
+            has = np.ones(1, np.uint8)
+            pb = np.ones(3, np.uint8)
+            c = np.array([rc])
+            r = C.c_void_p()
+            if lib.nl_search_systems(pos.ctypes.data_as(L.dp), pos.shape[0],
+                                     sz.ctypes.data_as(L.i64p), 1, H9.ctypes.data_as(L.dp),
+                                     has.ctypes.data_as(L.u8p), pb.ctypes.data_as(L.u8p),
+                                     c.ctypes.data_as(L.dp), 1, 0, 0, 1, 0, 1, C.byref(r)) != 0:
+                raise RuntimeError(L.last_error())
+            m = int(lib.nl_result_num_edges(r, 0))
+            ei = np.empty(m, np.int64)
+            ej = np.empty(m, np.int64)
+            s = np.empty((m, 3), np.int32)
+            d = np.empty(m)
+            lib.nl_result_copy(r, 0, ei.ctypes.data_as(L.i64p), ej.ctypes.data_as(L.i64p),
+                               s.ctypes.data_as(L.i32p))
+            lib.nl_result_copy_dist(r, 0, d.ctypes.data_as(L.dp))
+            lib.nl_result_free(r)
+            rij = pos[ej] + s @ np.asarray(H).T - pos[ei]
+            ref = np.sqrt((rij * rij).sum(1))
+            err = np.abs(d - ref)
+            key = np.lexsort((s[:, 2], s[:, 1], s[:, 0], ej, ei))
+            sets.append((ei[key], ej[key], s[key]))
+            out["fp32_filter" if f32 else "fp64"] = {
+                "edges": m, "max_abs_err_A": float(err.max()) if m else 0.0,
+                "max_rel_err": float((err / ref).max()) if m else 0.0,
+                "max_dist_A": float(d.max()) if m else 0.0}
+    finally:
+        lib.nl_set_fp32_filter(old)
+        lib.nl_set_autotune(old_tune)
+    same = all(np.array_equal(a, b) for a, b in zip(sets[0], sets[1]))
+    out["fp32_and_fp64_edge_sets_identical"] = bool(same)
+    out["reference"] = "float64 numpy |p_j + H S - p_i| from the raw positions"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +434,8 @@ def main():
         e3 = run_systems(lib, L, d3.ptr, p3.shape[0], s3, H3a, h1, p1, c3, False)
     c3_ms = 1e3 * (time.perf_counter() - t0) / k3
     log(f"c3 rc6: {c3_ms:.3f} ms, {e3} edges")
+    precision = precision_report(L, p3, H3, 6.0)
+    log(f"precision: {precision}")
 
     serving = None
     if args.serve_requests > 0:
@@ -448,6 +501,7 @@ def main():
                              "cpu_ms": None if c3_cpu_ms is None else round(c3_cpu_ms, 3),
                              "speedup_vs_cpu": None if c3_cpu_ms is None
                              else round(c3_cpu_ms / c3_ms, 2)},
+        "precision_c3_rc6": precision,
         "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
