@@ -456,7 +456,31 @@ struct WaveLds {
   int32_t bs[3][64];  // bin-image shift of the segment
   double off[3][64];  // H bs in FP64, reference order
   float off32[3][64]; // H bs in FP32 (filter)
+  // home atoms of the item, read at one uniform address per step (a
+  // broadcast LDS read instead of a readlane per coordinate)
+  union {
+    Pos64 home64[64];   // FP64 wrapped position, w = local index (FP64 kernels)
+    float4 home32[64];  // FP32 wrapped position, w = local index bits (FP32 kernels)
+  };
+  int4 homesi[64];      // wrap shifts, w = local index (fill pass)
 };
+
+// q = v / m, r = v % m for 0 <= v < 2^22, 1 <= m < 2^22: a float estimate
+// of the quotient (off by at most one there) and one exact correction step
+// (integer division takes several times the instructions).
+__device__ __forceinline__ void divmod_small(int v, int m, float inv_m, int* q, int* r) {
+  int qq = (int)((float)v * inv_m);
+  int rr = v - (int)__umul24((unsigned)qq, (unsigned)m);
+  if (rr < 0) {
+    qq -= 1;
+    rr += m;
+  } else if (rr >= m) {
+    qq += 1;
+    rr -= m;
+  }
+  *q = qq;
+  *r = rr;
+}
 
 // The lists handled by one launch: q_lo .. q_lo+cnt-1 in ascending-cutoff
 // order.  Held in registers for the whole wave.  For cumulative lists a
@@ -655,29 +679,49 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   const int slice = (T - tc + A.parts - 1) / A.parts;
   const int t_lo = tc + part * slice;
   const int t_hi = min(T, t_lo + slice);
+  // stencil decode by float reciprocals when every operand is small
+  const bool small = T < (1 << 21) && nbx + 2 * nsx < (1 << 20) && nby + 2 * nsy < (1 << 20) &&
+                     nbz + 2 * nsz < (1 << 20);
+  const float iwz = 1.0f / (float)wz, iwy = 1.0f / (float)wy;
+  const float inx = 1.0f / (float)nbx, iny = 1.0f / (float)nby, inz = 1.0f / (float)nbz;
+  // shift by whole periods so that the bin index is non-negative
+  const int kx = (nsx + nbx - 1) / nbx + 1, ky = (nsy + nby - 1) / nby + 1,
+            kz = (nsz + nbz - 1) / nbz + 1;
   for (int hb = hs; hb < he; hb += 64) {
-    const int hn = min(64, he - hb);
-    Pos64 hp = {0.0, 0.0, 0.0, 0.0};
-    int4 hsi = make_int4(0, 0, 0, 0);
-    float4 hp32 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int hn = uniform_i32(min(64, he - hb));
     if (lane < hn) {
-      if (!F32) hp = A.spos64[hb + lane];  // F32: read at a band recheck only
-      hsi = A.sidx[hb + lane];
-      if (F32) hp32 = A.spos32[hb + lane];
+      if (F32)
+        L.home32[lane] = A.spos32[hb + lane];
+      else
+        L.home64[lane] = A.spos64[hb + lane];
+      if (FILL) L.homesi[lane] = A.sidx[hb + lane];
     }
     for (int t0 = t_lo; t0 < t_hi; t0 += 64) {
       const int t = t0 + lane;
       int len = 0, st = 0, bsx = 0, bsy = 0, bsz = 0;
       double ox = 0, oy = 0, oz = 0;
       if (t < t_hi) {
-        const int dz = t % wz - nsz;
-        const int rest = t / wz;
-        const int dy = rest % wy - nsy;
-        const int dx = rest / wy - nsx;
         int cx, cy, cz;
-        div_mod(bx + dx, nbx, &cx, &bsx);
-        div_mod(by + dy, nby, &cy, &bsy);
-        div_mod(bz + dz, nbz, &cz, &bsz);
+        if (small) {
+          int rest, dz, dy, dx, r;
+          divmod_small(t, wz, iwz, &rest, &dz);
+          divmod_small(rest, wy, iwy, &dx, &dy);
+          divmod_small(bx + dx - nsx + kx * nbx, nbx, inx, &bsx, &cx);
+          divmod_small(by + dy - nsy + ky * nby, nby, iny, &bsy, &cy);
+          divmod_small(bz + dz - nsz + kz * nbz, nbz, inz, &bsz, &cz);
+          bsx -= kx;
+          bsy -= ky;
+          bsz -= kz;
+          (void)r;
+        } else {
+          const int dz = t % wz - nsz;
+          const int rest = t / wz;
+          const int dy = rest % wy - nsy;
+          const int dx = rest / wy - nsx;
+          div_mod(bx + dx, nbx, &cx, &bsx);
+          div_mod(by + dy, nby, &cy, &bsy);
+          div_mod(bz + dz, nbz, &cz, &bsz);
+        }
         const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
         st = A.cell_start[bin];
         const int en = A.cell_start[bin + 1];
@@ -768,19 +812,27 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             pz = c32.z + L.off32[2][csg];
           }
           const int oj = F32 ? __float_as_int(c32.w) : (int)c.w;
+          // home atom i is tested against this lane's candidate iff i < lim
+          // (in the home bin without image only the later slots)
+          const int lim = cvalid ? (home_seg ? cjs - hb : 64) : 0;
 #pragma unroll 1
           for (int i = 0; i < hn; ++i) {
-            const int oi = __builtin_amdgcn_readlane(hsi.w, i);
-            const bool cand = cvalid && (!home_seg || cjs > hb + i);
+            // broadcast LDS reads of the home atom (uniform address)
+            float4 h32 = make_float4(0.f, 0.f, 0.f, 0.f);
+            Pos64 h64 = {0.0, 0.0, 0.0, 0.0};
+            if (F32)
+              h32 = L.home32[i];
+            else
+              h64 = L.home64[i];
+            const int oi = F32 ? __float_as_int(h32.w) : (int)h64.w;
+            const bool cand = i < lim;
             if (!FILL && !C.disjoint) {
               // Count pass, cumulative lists: lane counters, no ballots.  A
               // self image (same atom, other image) counts twice (both S and
               // -S are emitted).  Lists past q_cnt have -inf thresholds.
               const uint32_t w1 = cand ? 1u + (uint32_t)(oj == oi) : 0u;
               if (F32) {
-                const float xi = readlane_f(hp32.x, i), yi = readlane_f(hp32.y, i),
-                            zi = readlane_f(hp32.z, i);
-                const float dx = px - xi, dy = py - yi, dz = pz - zi;
+                const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
                 const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 bool unsure = false;
 #pragma unroll
@@ -804,10 +856,9 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                   }
                 }
               } else {
-                const double xi = readlane_d(hp.x, i), yi = readlane_d(hp.y, i),
-                             zi = readlane_d(hp.z, i);
                 // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-                const double d2 = norm2((c.x - xi) + cox, (c.y - yi) + coy, (c.z - zi) + coz);
+                const double d2 =
+                    norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
 #pragma unroll
                 for (int g = 0; g < NQ; ++g) cur[g] += d2 < C.r[g] ? w1 : 0u;
               }
@@ -816,9 +867,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             uint32_t mem = 0;
             double d2 = INFINITY;
             if (F32) {
-              const float xi = readlane_f(hp32.x, i), yi = readlane_f(hp32.y, i),
-                          zi = readlane_f(hp32.z, i);
-              const float dx = px - xi, dy = py - yi, dz = pz - zi;
+              const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
               const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
               bool unsure = false;
               mem = members32(e2, C, &unsure);
@@ -835,10 +884,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                 }
               }
             } else {
-              const double xi = readlane_d(hp.x, i), yi = readlane_d(hp.y, i),
-                           zi = readlane_d(hp.z, i);
               // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-              d2 = norm2((c.x - xi) + cox, (c.y - yi) + coy, (c.z - zi) + coz);
+              d2 = norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
               if (cand) mem = members64(d2, C);
             }
             if (FILL && __ballot(mem != 0) == 0) continue;
@@ -846,9 +893,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             int sx = 0, sy = 0, sz = 0;
             int64_t gi = 0, gj = 0;
             if (FILL) {
-              const int six = __builtin_amdgcn_readlane(hsi.x, i);
-              const int siy = __builtin_amdgcn_readlane(hsi.y, i);
-              const int siz = __builtin_amdgcn_readlane(hsi.z, i);
+              const int4 hsi = L.homesi[i];
+              const int six = hsi.x, siy = hsi.y, siz = hsi.z;
               if (mem) {
                 sx = cj.x - six + sbx;
                 sy = cj.y - siy + sby;
