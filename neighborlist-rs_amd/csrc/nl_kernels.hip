@@ -687,6 +687,12 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   // shift by whole periods so that the bin index is non-negative
   const int kx = (nsx + nbx - 1) / nbx + 1, ky = (nsy + nby - 1) / nby + 1,
             kz = (nsz + nbz - 1) / nbz + 1;
+  // stencil pruning: perpendicular bin widths, cell shape, bound on d^2
+  const float bwx = (float)(d->w[0] / (double)nbx), bwy = (float)(d->w[1] / (double)nby),
+              bwz = (float)(d->w[2] / (double)nbz);
+  const bool diag = d->h[1] == 0.0 && d->h[2] == 0.0 && d->h[3] == 0.0 && d->h[5] == 0.0 &&
+                    d->h[6] == 0.0 && d->h[7] == 0.0;
+  const float prune2 = (float)(d->rc_max * d->rc_max * (1.0 + 1e-4));
   for (int hb = hs; hb < he; hb += 64) {
     const int hn = uniform_i32(min(64, he - hb));
     if (lane < hn) {
@@ -702,31 +708,49 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
       double ox = 0, oy = 0, oz = 0;
       if (t < t_hi) {
         int cx, cy, cz;
+        int ox_, oy_, oz_;  // stencil offset in bins
         if (small) {
-          int rest, dz, dy, dx, r;
+          int rest, dz, dy, dx;
           divmod_small(t, wz, iwz, &rest, &dz);
           divmod_small(rest, wy, iwy, &dx, &dy);
-          divmod_small(bx + dx - nsx + kx * nbx, nbx, inx, &bsx, &cx);
-          divmod_small(by + dy - nsy + ky * nby, nby, iny, &bsy, &cy);
-          divmod_small(bz + dz - nsz + kz * nbz, nbz, inz, &bsz, &cz);
+          ox_ = dx - nsx;
+          oy_ = dy - nsy;
+          oz_ = dz - nsz;
+          divmod_small(bx + ox_ + kx * nbx, nbx, inx, &bsx, &cx);
+          divmod_small(by + oy_ + ky * nby, nby, iny, &bsy, &cy);
+          divmod_small(bz + oz_ + kz * nbz, nbz, inz, &bsz, &cz);
           bsx -= kx;
           bsy -= ky;
           bsz -= kz;
-          (void)r;
         } else {
-          const int dz = t % wz - nsz;
+          oz_ = t % wz - nsz;
           const int rest = t / wz;
-          const int dy = rest % wy - nsy;
-          const int dx = rest / wy - nsx;
-          div_mod(bx + dx, nbx, &cx, &bsx);
-          div_mod(by + dy, nby, &cy, &bsy);
-          div_mod(bz + dz, nbz, &cz, &bsz);
+          oy_ = rest % wy - nsy;
+          ox_ = rest / wy - nsx;
+          div_mod(bx + ox_, nbx, &cx, &bsx);
+          div_mod(by + oy_, nby, &cy, &bsy);
+          div_mod(bz + oz_, nbz, &cz, &bsz);
         }
-        const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
-        st = A.cell_start[bin];
-        const int en = A.cell_start[bin + 1];
-        if (t == tc) st = hb;  // home bin, no image: slots from this group on
-        len = en - st;
+        // Bins that cannot hold a partner: a home atom and a candidate
+        // |o_k| bins apart along lattice axis k differ by at least
+        // (|o_k| - 1) / nb_k in fractional coordinate k, hence by at least
+        // (|o_k| - 1) w_k / nb_k in distance (w_k = 1/|row_k(H^-1)|).  The
+        // per-axis bounds add in quadrature for a diagonal cell; otherwise
+        // the largest one is the bound.  Evaluated in FP32 against a bound
+        // widened by 1e-4 (relative), far beyond the FP32 rounding here and
+        // that of the wrapped positions.
+        const float gx = (float)max(0, abs(ox_) - 1) * bwx;
+        const float gy = (float)max(0, abs(oy_) - 1) * bwy;
+        const float gz = (float)max(0, abs(oz_) - 1) * bwz;
+        const float lb2 = diag ? (gx * gx + gy * gy) + gz * gz
+                               : fmaxf(fmaxf(gx * gx, gy * gy), gz * gz);
+        if (lb2 <= prune2) {
+          const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
+          st = A.cell_start[bin];
+          const int en = A.cell_start[bin + 1];
+          if (t == tc) st = hb;  // home bin, no image: slots from this group on
+          len = en - st;
+        }
         // (ox + oy) + oz of search.rs:498 = H (bsx, bsy, bsz) in gemv order
         ox = (h0 * (double)bsx + h1 * (double)bsy) + h2 * (double)bsz;
         oy = (h3 * (double)bsx + h4 * (double)bsy) + h5 * (double)bsz;
