@@ -118,9 +118,10 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   for (int s = 0; s <= S; ++s) item_base[s] = starts[s] * (1 + tune.parts);
   scan_len = (int64_t)K * n_items;
   stiles = radix_tiles(scan_len);
-  // one zero-initialised block: scan look-back status (u64, first), radix
-  // histograms, radix look-back status, tile tickets
-  z_words = 2 * stiles + 256 * passes + 256 * rtiles * passes + 8;
+  // one zero-initialised block: count-scan look-back status (u64, first),
+  // radix digit-count scan status (u64, one block per pass), tickets
+  rstiles = radix_tiles(256 * rtiles);
+  z_words = 2 * stiles + 2 * rstiles * passes + 16;
 
   prm = Params{};
   prm.n_atoms = n;
@@ -155,6 +156,7 @@ std::vector<size_t> Pipeline::buffer_sizes() const {
       8 * (size_t)(scan_len + 1),                                   // offsets
       8 * (size_t)(K + 1),                                          // bounds
       8 * (size_t)n_items,                                          // work list
+      radix_bytes(),  // radix scratch: digit counts, their offsets, bounds
   };
 }
 
@@ -178,6 +180,9 @@ void Pipeline::assign(Buffers& b, const std::vector<void*>& p) const {
   b.offsets = (int64_t*)p[16];
   b.bounds = (int64_t*)p[17];
   b.list = (int64_t*)p[18];
+  b.rcnt = (uint32_t*)p[19];
+  b.roff = (int64_t*)((char*)p[19] + roff_at());
+  b.rbounds = b.roff + 256 * rtiles + 1;
 }
 
 size_t Pipeline::pinned_bytes() const { return off_bounds() + 8 * (size_t)(K + 1) + 64; }
@@ -222,7 +227,7 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.counts = b.counts;
   A.offsets = b.offsets;
   A.list = b.list;
-  A.n_real = b.zero + 2 * P.stiles + 256 * P.passes + 256 * P.rtiles * P.passes + 6;
+  A.n_real = b.zero + 2 * P.stiles + 2 * P.rstiles * P.passes + 6;
   A.bounds = b.bounds;
   A.n = P.n_items;
   A.n_sys = P.S;
@@ -238,17 +243,16 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
 
 hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const {
   uint64_t* sstatus = (uint64_t*)b.zero;
-  uint32_t* hist = b.zero + 2 * stiles;
-  uint32_t* rstatus = hist + 256 * passes;
-  uint32_t* tickets = rstatus + 256 * rtiles * passes;
+  uint64_t* rstatus = (uint64_t*)(b.zero + 2 * stiles);
+  uint32_t* tickets = b.zero + 2 * stiles + 2 * rstiles * passes;
   // zero block and counts (items without work keep zero counts)
   NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st));
   NL_TRY(launch_plan(b.pos, b.sys, b.band, b.rc2, prm, st));
-  NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, hist, passes, st));
+  NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, st));
   uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
   for (int ps = 0; ps < passes; ++ps) {
-    NL_TRY(launch_radix_pass(kin, vin, kout, vout, n, kRadixBits * ps, hist + 256 * ps,
-                             rstatus + (int64_t)256 * rtiles * ps, tickets + ps, st));
+    NL_TRY(launch_radix_pass(kin, vin, kout, vout, n, kRadixBits * ps, b.rcnt, b.roff,
+                             b.rbounds, rstatus + rstiles * ps, tickets + ps, st));
     std::swap(kin, kout);
     std::swap(vin, vout);
   }

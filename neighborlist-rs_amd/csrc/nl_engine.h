@@ -78,8 +78,11 @@ struct Buffers {
   int64_t* offsets = nullptr;
   int64_t* bounds = nullptr;
   int64_t* list = nullptr;  // work items that have work (any order)
+  uint32_t* rcnt = nullptr;    // radix: digit counts per tile [256][tiles]
+  int64_t* roff = nullptr;     // radix: their exclusive offsets [256 * tiles + 1]
+  int64_t* rbounds = nullptr;  // radix: scan bounds [2]
 };
-constexpr int kNumBuffers = 19;
+constexpr int kNumBuffers = 20;
 
 // Host-prepared launch sequence for one request shape.
 class Pipeline {
@@ -103,7 +106,7 @@ class Pipeline {
   int rank_of(int list) const;  // input list -> sorted rank
 
   int64_t n = 0, n_items = 0, bin_total = 0, scan_len = 0, rtiles = 0, stiles = 0,
-          z_words = 0;
+          rstiles = 0, z_words = 0;
   int S = 0, K = 0, passes = 0;
   Tuning tune;
   Params prm{};
@@ -114,6 +117,9 @@ class Pipeline {
   std::vector<SysDesc> desc;
   std::vector<int64_t> starts, item_base;
 
+  // radix scratch block: counts (u32), then offsets and bounds (i64)
+  size_t roff_at() const { return (4 * 256 * (size_t)rtiles + 7) & ~size_t(7); }
+  size_t radix_bytes() const { return roff_at() + 8 * (256 * (size_t)rtiles + 3); }
   // layout of the descriptor block (pinned staging and device alike)
   size_t off_start() const { return sizeof(SysDesc) * S; }
   size_t off_items() const { return off_start() + 8 * (size_t)(S + 1); }

@@ -42,11 +42,13 @@ hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStr
 hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
                        const Params& p, hipStream_t st);
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
-                      int64_t n, uint32_t* keys, uint32_t* vals, uint32_t* hist, int passes,
-                      hipStream_t st);
+                      int64_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
+// one stable 8-bit pass: cnt [256 * tiles] u32, off [256 * tiles + 1] i64,
+// bounds [2] i64 and a zeroed scan status [radix_tiles(256 * tiles)] + ticket
 hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                             uint32_t* vout, int64_t n, int shift, const uint32_t* hist,
-                             uint32_t* status, uint32_t* ticket, hipStream_t st);
+                             uint32_t* vout, int64_t n, int shift, uint32_t* cnt, int64_t* off,
+                             int64_t* bounds, uint64_t* status, uint32_t* ticket,
+                             hipStream_t st);
 hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const double* pos,
                            const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,
                            int64_t total_bins, int32_t* cell_start, Pos64* spos64,

@@ -3,10 +3,9 @@
 // Pipeline per call (one launch per stage for the whole ragged batch):
 //   plan      one block per system: auto-box span, H^-1, widths, dispatch
 //             (single.rs:19-89), GPU bin grid, FP32 band thresholds
-//   bin       one thread per atom: wrapped fractional coordinate -> bin key,
-//             global digit histograms for the radix passes
-//   radix     LSD radix sort of (key, atom) pairs, 8-bit digits, one launch
-//             per digit with decoupled look-back between tiles (stable)
+//   bin       one thread per atom: wrapped fractional coordinate -> bin key
+//   radix     stable LSD radix sort of (key, atom) pairs, 8-bit digits; per
+//             digit: tile counts, one scan over (digit, tile), scatter
 //   finalize  one thread per sorted slot: cell offsets from key boundaries,
 //             sorted wrapped positions (FP64 + FP32) and wrap shifts
 //   pairs     one wavefront per unit (a brute-force row or a sorted atom):
@@ -280,67 +279,72 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ pos,
                                              const SysDesc* __restrict__ sys,
                                              const int64_t* __restrict__ sys_start, int n_sys,
                                              int64_t n, uint32_t* __restrict__ keys,
-                                             uint32_t* __restrict__ vals,
-                                             uint32_t* __restrict__ hist, int passes) {
-  __shared__ uint32_t lh[4][256];
-  for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) (&lh[0][0])[i] = 0;
-  __syncthreads();
+                                             uint32_t* __restrict__ vals) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < n) {
-    const int s = find_sys(sys_start, n_sys, a);
-    const SysDesc* d = sys + s;
-    uint32_t key = (uint32_t)d->bin_base;
-    if (d->mode == kCellList) {
-      const double* q = pos + 3 * a;
-      double f[3];
-      matvec3(d->hinv, q[0], q[1], q[2], f);
-      int b[3];
+  if (a >= n) return;
+  const int s = find_sys(sys_start, n_sys, a);
+  const SysDesc* d = sys + s;
+  uint32_t key = (uint32_t)d->bin_base;
+  if (d->mode == kCellList) {
+    const double* q = pos + 3 * a;
+    double f[3];
+    matvec3(d->hinv, q[0], q[1], q[2], f);
+    int b[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double u = f[k] - floor(f[k]);
-        int v = (int)(u * (double)d->nb[k]);  // u >= 0
-        b[k] = v < d->nb[k] - 1 ? v : d->nb[k] - 1;
-      }
-      key += (uint32_t)(b[0] + d->nb[0] * (b[1] + d->nb[1] * b[2]));
+    for (int k = 0; k < 3; ++k) {
+      double u = f[k] - floor(f[k]);
+      int v = (int)(u * (double)d->nb[k]);  // u >= 0
+      b[k] = v < d->nb[k] - 1 ? v : d->nb[k] - 1;
     }
-    keys[a] = key;
-    vals[a] = (uint32_t)a;
-    for (int ps = 0; ps < passes; ++ps) atomicAdd(&lh[ps][(key >> (kRadixBits * ps)) & 255u], 1u);
+    key += (uint32_t)(b[0] + d->nb[0] * (b[1] + d->nb[1] * b[2]));
   }
-  __syncthreads();
-  for (int ps = 0; ps < passes; ++ps) {
-    uint32_t c = lh[ps][threadIdx.x];
-    if (c) atomicAdd(&hist[ps * 256 + threadIdx.x], c);
-  }
+  keys[a] = key;
+  vals[a] = (uint32_t)a;
 }
 
 // ---------------------------------------------------------------------------
-// One stable LSD radix pass over 8 bits.  Tile = 2048 keys; tiles are taken
-// in ticket order, so every tile a block waits on has already published its
-// per-digit count and the look-back always terminates.
-__global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__ kin,
-                                                    const uint32_t* __restrict__ vin,
-                                                    uint32_t* __restrict__ kout,
-                                                    uint32_t* __restrict__ vout, int64_t n,
-                                                    int shift, const uint32_t* __restrict__ hist,
-                                                    uint32_t* __restrict__ status,
-                                                    uint32_t* __restrict__ ticket) {
-  __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_base[256];
+// Stable LSD radix sort, one 8-bit digit per pass, reduce-then-scan: per
+// tile (2048 keys) digit counts, one exclusive scan over (digit, tile) in
+// digit-major order (the scan kernel below), then a stable scatter.  No
+// block waits on another, so a pass costs three short launches instead of a
+// chain of look-backs across tiles.
+//
+// Digit counts of one tile: cnt[digit * tiles + tile].
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ kin, int64_t n,
+                                                    int shift, int64_t tiles,
+                                                    uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[256];
+  const int tid = threadIdx.x;
+  h[tid] = 0u;
+  __syncthreads();
+  const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int64_t e = t0 + r * 256 + tid;
+    if (e < n) atomicAdd(&h[(kin[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  cnt[(int64_t)tid * tiles + blockIdx.x] = h[tid];
+}
+
+// Scatter of one tile: keys are ranked within the tile in element order
+// (8 rounds of 256, waves in order, lanes in order), so equal digits keep
+// their input order; off[digit * tiles + tile] is where the tile's keys of
+// that digit start.
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout, int64_t n,
+                                                       int shift, int64_t tiles,
+                                                       const int64_t* __restrict__ off) {
   __shared__ uint32_t s_run[256];
   __shared__ uint32_t s_cnt[4][256];
-  __shared__ uint32_t s_excl[256];
-  __shared__ uint32_t s_tmp[4];
   const int tid = threadIdx.x, w = tid >> 6;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  const uint32_t hv = hist[tid];
-  const uint32_t base_d = block_excl_scan_256(hv, s_tmp, nullptr);
-  s_base[tid] = base_d;
   s_run[tid] = 0;
   for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
-  const int64_t t0 = (int64_t)tile * kRadixTile;
+  const int64_t tile = blockIdx.x;
+  const int64_t t0 = tile * kRadixTile;
   uint32_t kk[8], vv[8], lp[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
@@ -370,35 +374,12 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     s_cnt[0][tid] = s_cnt[1][tid] = s_cnt[2][tid] = s_cnt[3][tid] = 0;
     __syncthreads();
   }
-  // decoupled look-back, one digit per thread
-  {
-    const uint32_t cnt = s_run[tid];
-    uint32_t* st = status + (int64_t)tile * 256 + tid;
-    uint32_t excl = 0;
-    if (tile == 0) {
-      atomic_store_agent(st, kFlagPre | cnt);
-    } else {
-      atomic_store_agent(st, kFlagAgg | cnt);
-      int64_t t = (int64_t)tile - 1;
-      while (true) {
-        const uint32_t v = atomic_load_agent(status + t * 256 + tid);
-        const uint32_t flag = v & ~kValMask;
-        if (flag == 0) continue;  // predecessor still ranking its tile
-        excl += v & kValMask;
-        if (flag == kFlagPre) break;
-        --t;
-      }
-      atomic_store_agent(st, kFlagPre | (excl + cnt));
-    }
-    s_excl[tid] = excl;
-  }
-  __syncthreads();
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int64_t e = t0 + r * 256 + tid;
     if (e < n) {
       const uint32_t dig = (kk[r] >> shift) & 255u;
-      const uint32_t dst = s_base[dig] + s_excl[dig] + lp[r];
+      const int64_t dst = off[(int64_t)dig * tiles + tile] + lp[r];
       kout[dst] = kk[r];
       vout[dst] = vv[r];
     }
@@ -1188,20 +1169,26 @@ hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const doub
 }
 
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
-                      int64_t n, uint32_t* keys, uint32_t* vals, uint32_t* hist, int passes,
-                      hipStream_t st) {
+                      int64_t n, uint32_t* keys, uint32_t* vals, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bin, dim3(grid_for(n, 256)), dim3(256), 0, st, pos, sys, sys_start, n_sys,
-                     n, keys, vals, hist, passes);
+                     n, keys, vals);
   return hipGetLastError();
 }
 
 hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                             uint32_t* vout, int64_t n, int shift, const uint32_t* hist,
-                             uint32_t* status, uint32_t* ticket, hipStream_t st) {
+                             uint32_t* vout, int64_t n, int shift, uint32_t* cnt, int64_t* off,
+                             int64_t* bounds, uint64_t* status, uint32_t* ticket,
+                             hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_radix_pass, dim3((unsigned)radix_tiles(n)), dim3(256), 0, st, kin, vin,
-                     kout, vout, n, shift, hist, status, ticket);
+  const int64_t tiles = radix_tiles(n);
+  hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(256), 0, st, kin, n, shift, tiles,
+                     cnt);
+  const hipError_t e = launch_scan(cnt, off, 256 * tiles, 256 * tiles, 1, bounds, status, ticket,
+                                   st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(256), 0, st, kin, vin, kout,
+                     vout, n, shift, tiles, (const int64_t*)off);
   return hipGetLastError();
 }
 
