@@ -327,25 +327,17 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
   cnt[(int64_t)tid * tiles + blockIdx.x] = h[tid];
 }
 
-// Scatter of one tile: keys are ranked within the tile in element order
+// Ranks the keys of one tile (2048, 256 threads) by digit in element order
 // (8 rounds of 256, waves in order, lanes in order), so equal digits keep
-// their input order; off[digit * tiles + tile] is where the tile's keys of
-// that digit start.
-__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                       const uint32_t* __restrict__ vin,
-                                                       uint32_t* __restrict__ kout,
-                                                       uint32_t* __restrict__ vout, int64_t n,
-                                                       int shift, int64_t tiles,
-                                                       const int64_t* __restrict__ off) {
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_cnt[4][256];
+// their input order: lp[r] = s_run[digit] on entry + rank within the tile.
+// s_run[d] advances by the tile's count of digit d; s_cnt is zero on entry
+// and on exit.
+__device__ __forceinline__ void radix_rank_tile(const uint32_t* __restrict__ kin,
+                                                const uint32_t* __restrict__ vin, int64_t n,
+                                                int shift, int64_t t0, uint32_t* s_run,
+                                                uint32_t (*s_cnt)[256], uint32_t* kk,
+                                                uint32_t* vv, uint32_t* lp) {
   const int tid = threadIdx.x, w = tid >> 6;
-  s_run[tid] = 0;
-  for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
-  __syncthreads();
-  const int64_t tile = blockIdx.x;
-  const int64_t t0 = tile * kRadixTile;
-  uint32_t kk[8], vv[8], lp[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int64_t e = t0 + r * 256 + tid;
@@ -374,6 +366,26 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     s_cnt[0][tid] = s_cnt[1][tid] = s_cnt[2][tid] = s_cnt[3][tid] = 0;
     __syncthreads();
   }
+}
+
+// Scatter of one tile: off[digit * tiles + tile] is where the tile's keys
+// of that digit start.
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout, int64_t n,
+                                                       int shift, int64_t tiles,
+                                                       const int64_t* __restrict__ off) {
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_cnt[4][256];
+  const int tid = threadIdx.x;
+  s_run[tid] = 0;
+  for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  const int64_t t0 = tile * kRadixTile;
+  uint32_t kk[8], vv[8], lp[8];
+  radix_rank_tile(kin, vin, n, shift, t0, s_run, s_cnt, kk, vv, lp);
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int64_t e = t0 + r * 256 + tid;
@@ -382,6 +394,41 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
       const int64_t dst = off[(int64_t)dig * tiles + tile] + lp[r];
       kout[dst] = kk[r];
       vout[dst] = vv[r];
+    }
+  }
+}
+
+// A whole pass in one block for small inputs (a few tiles): digit totals,
+// their exclusive scan, then the tiles ranked in order with running digit
+// positions.  One launch instead of three.
+__global__ __launch_bounds__(256) void k_radix_small(const uint32_t* __restrict__ kin,
+                                                     const uint32_t* __restrict__ vin,
+                                                     uint32_t* __restrict__ kout,
+                                                     uint32_t* __restrict__ vout, int64_t n,
+                                                     int shift) {
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_cnt[4][256];
+  __shared__ uint32_t s_tmp[4];
+  const int tid = threadIdx.x;
+  s_run[tid] = 0;
+  for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
+  __syncthreads();
+  for (int64_t e = tid; e < n; e += 256) atomicAdd(&s_run[(kin[e] >> shift) & 255u], 1u);
+  __syncthreads();
+  const uint32_t tot = s_run[tid];
+  const uint32_t base = block_excl_scan_256(tot, s_tmp, nullptr);
+  s_run[tid] = base;
+  __syncthreads();
+  for (int64_t t0 = 0; t0 < n; t0 += kRadixTile) {
+    uint32_t kk[8], vv[8], lp[8];
+    radix_rank_tile(kin, vin, n, shift, t0, s_run, s_cnt, kk, vv, lp);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int64_t e = t0 + r * 256 + tid;
+      if (e < n) {
+        kout[lp[r]] = kk[r];
+        vout[lp[r]] = vv[r];
+      }
     }
   }
 }
@@ -1182,6 +1229,11 @@ hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t*
                              hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const int64_t tiles = radix_tiles(n);
+  if (tiles <= 2) {  // one block ranks the tiles in order: one launch
+    // (one block is slower than three launches from about 3 tiles on)
+    hipLaunchKernelGGL(k_radix_small, dim3(1), dim3(256), 0, st, kin, vin, kout, vout, n, shift);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(256), 0, st, kin, n, shift, tiles,
                      cnt);
   const hipError_t e = launch_scan(cnt, off, 256 * tiles, 256 * tiles, 1, bounds, status, ticket,

@@ -255,3 +255,28 @@ def test_parallel_flag_does_not_change_results(nr, oracle):
     b = as_tuple(nr.build_neighborlists(cell, p, 6.0, parallel=False))
     assert_same_edges(a, b, "parallel vs serial")
     assert_same_edges(b, oracle.search_single(p, H, [1, 1, 1], 6.0, parallel=False), "serial")
+
+
+def test_full_ragged_batch_vs_oracle(nr, oracle):
+    """The full 256-system batch of the benchmark (113,559 atoms, 56 radix tiles:
+    the multi-tile sort path), cutoffs {5, 6, 14} in one pass."""
+    from neighborlist_rs import workloads as W
+    p, sizes, cells, cut = W.c4_ragged_batch()
+    batch, rows = W.batch_arrays(sizes, cells)
+    _print_case("full ragged batch", len(sizes), p.shape[0])
+    ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+    want = oracle.search_batch_multi(p, batch, ocells, list(cut), False)
+    got = nr.build_neighborlists_batch_multi(p, batch, rows, list(cut))
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"list {k}")
+
+
+def test_large_single_system_vs_oracle(nr, oracle):
+    """One periodic system of 60,000 atoms (30 radix tiles), rc 5."""
+    from neighborlist_rs import workloads as W
+    rng = np.random.default_rng(60)
+    p, H = W.random_system(rng, 60000, 2)
+    _print_case("large single", p.shape[0])
+    want = oracle.search_single(p, H, [1, 1, 1], 5.0)
+    got = _gpu_single(nr, p, H, [True] * 3, 5.0)
+    assert_same_edges(got, want, "60k triclinic rc=5")
