@@ -284,6 +284,7 @@ hipError_t Pipeline::enqueue_fill(const Buffers& b, void* edges, void* shifts, v
     A.q_cnt = std::min(kGroup, K - q0);
     NL_TRY(launch_pairs(true, A, st));
   }
+  if (dist) NL_TRY(launch_sqrt_dist((double*)dist, b.bounds + K, capacity, st));
   return hipSuccess;
 }
 

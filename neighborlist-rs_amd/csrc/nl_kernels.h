@@ -28,7 +28,7 @@ struct PairArgs {
   const int64_t* bounds;     // [k + 1] first edge of each list (sorted-cutoff order)
   int64_t* edges;            // list q: sources at base_q + e, targets at base_q + M_q + e
   int32_t* shifts;           // list q: 3 per edge at 3 e
-  double* dist;              // optional |r_ij| per edge (FP64), list q at e
+  double* dist;              // optional per edge: d^2 from the fill pass, then |r_ij|
   int64_t n;                 // work items
   int64_t capacity;          // edges the output buffers hold (fill pass)
   int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index;
@@ -39,6 +39,8 @@ struct PairArgs {
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st);
+hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity,
+                            hipStream_t st);
 hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
                        const Params& p, hipStream_t st);
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,

@@ -40,11 +40,6 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
 }
 
 __device__ inline int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline int64_t uniform_i64(int64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
 
 __device__ inline int find_sys(const int64_t* sys_start, int n_sys, int64_t a) {
   int lo = 0, hi = n_sys - 1;
@@ -99,6 +94,18 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+// Distances: the fill pass stores d^2 (FP64, the reference expression);
+// here |r_ij| = sqrt(d^2), for the edges actually written (none when the
+// lists did not fit the output buffers).
+__global__ void k_sqrt_dist(double* __restrict__ dist, const int64_t* __restrict__ total,
+                            int64_t capacity) {
+  const int64_t m = *total;
+  if (m > capacity) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+    dist[i] = sqrt(dist[i]);
 }
 
 // Clears two word ranges in one launch (a kernel, not a memset node, so the
@@ -571,7 +578,7 @@ using Acc = typename std::conditional<FILL, int64_t, uint32_t>::type;
 // list bits (zero for lanes without a pair).
 template <bool FILL, int NQ>
 __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
-                                 int64_t gj, int sx, int sy, int sz, double dd, Acc<FILL>* cur) {
+                                 int64_t gj, int sx, int sy, int sz, double d2, Acc<FILL>* cur) {
   if (!FILL) {
 #pragma unroll
     for (int g = 0; g < NQ; ++g) cur[g] += (mem >> g) & 1u;
@@ -584,23 +591,19 @@ __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, 
     const uint64_t m = __ballot(on);
     if (m == 0) continue;
     if (FILL && on) {
-      // wave-uniform row pointers plus a 32-bit lane offset: the stores
-      // take the scalar-base addressing form
-      const uint32_t lb = lanes_below(m);
-      int64_t* pi = A.edges + (C.base[g] + cur[g]);
-      int64_t* pj = pi + C.len[g];
-      int32_t* ps = A.shifts + 3 * cur[g];
       // plain stores: L2 merges the partial lines of successive steps
       // (non-temporal stores measured 10% slower here)
-      pi[lb] = gi;
-      pj[lb] = gj;
-      ps[3 * lb] = sx;
-      ps[3 * lb + 1] = sy;
-      ps[3 * lb + 2] = sz;
-      if (A.dist) A.dist[cur[g] + lb] = dd;
+      const int64_t e = cur[g] + lanes_below(m);
+      int64_t* pi = A.edges + (C.base[g] + e);
+      pi[0] = gi;
+      pi[C.len[g]] = gj;
+      int32_t* q = A.shifts + 3 * e;  // one address, three offsets
+      q[0] = sx;
+      q[1] = sy;
+      q[2] = sz;
+      if (A.dist) A.dist[e] = d2;  // squared here; square roots in a pass of their own
     }
-    // the cursor is wave-uniform: keep it in scalar registers
-    cur[g] = uniform_i64(cur[g] + (int64_t)__popcll(m));
+    cur[g] += (int64_t)__popcll(m);
   }
 }
 
@@ -655,7 +658,7 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
     const uint32_t mem = valid ? members64(d2, C) : 0u;
     if (FILL && __ballot(mem != 0) == 0) continue;
     const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
-    emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, A.dist ? sqrt(d2) : 0.0, cur);
+    emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
   }
 }
 
@@ -963,10 +966,9 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                 gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
               }
             }
-            const double dd = A.dist ? sqrt(d2) : 0.0;
-            emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, dd, cur);
+            emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
             if (!FILL || __ballot(self_img))
-              emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, dd, cur);
+              emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, d2, cur);
           }
         }
       }
@@ -1197,6 +1199,15 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st) {
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity,
+                            hipStream_t st) {
+  if (capacity <= 0) return hipSuccess;
+  int64_t g = (capacity + 255) / 256;
+  if (g > 2048) g = 2048;
+  hipLaunchKernelGGL(k_sqrt_dist, dim3((unsigned)g), dim3(256), 0, st, dist, total, capacity);
   return hipGetLastError();
 }
 
