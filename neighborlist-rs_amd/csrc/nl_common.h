@@ -15,7 +15,8 @@ namespace nl {
 constexpr int kMaxCutoffs = 64;     // cutoffs per call (the reference has no limit; 64 is ample)
 constexpr int kGroup = 8;           // cutoffs classified per pair-kernel launch
 constexpr int kWaves = 4;           // waves (units) per pair-kernel block
-constexpr int kRadixBits = 8;       // LSD radix digit width
+constexpr int kRadixMaxBits = 12;   // LSD radix digit width limit (4096 digits)
+constexpr int kRadixMaxDigits = 1 << kRadixMaxBits;
 constexpr int kRadixTile = 2048;    // keys per radix / scan tile (256 threads x 8)
 
 // Search mode chosen per system, exactly as single.rs:60-89 / batch.rs:100-124.

@@ -105,7 +105,12 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
     set_error("request too large for one launch (more than 2^28 atoms or 2^31 bins)");
     return 1;
   }
-  passes = (bits_for(bin_total - 1) + kRadixBits - 1) / kRadixBits;
+  {
+    // fewest passes of at most 12-bit digits, digits as even as possible
+    const int kb = bits_for(bin_total - 1);
+    passes = (kb + kRadixMaxBits - 1) / kRadixMaxBits;
+    digit_bits = (kb + passes - 1) / passes;
+  }
   rtiles = radix_tiles(n);
   // work items: per system its atoms (brute-force rows), then
   // (slot, stencil slice) pairs (cell-list home groups)
@@ -120,7 +125,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   stiles = radix_tiles(scan_len);
   // one zero-initialised block: count-scan look-back status (u64, first),
   // radix digit-count scan status (u64, one block per pass), tickets
-  rstiles = radix_tiles(256 * rtiles);
+  rstiles = radix_tiles(((int64_t)1 << digit_bits) * rtiles);
   z_words = 2 * stiles + 2 * rstiles * passes + 16;
 
   prm = Params{};
@@ -182,7 +187,7 @@ void Pipeline::assign(Buffers& b, const std::vector<void*>& p) const {
   b.list = (int64_t*)p[18];
   b.rcnt = (uint32_t*)p[19];
   b.roff = (int64_t*)((char*)p[19] + roff_at());
-  b.rbounds = b.roff + 256 * rtiles + 1;
+  b.rbounds = b.roff + ((int64_t)1 << digit_bits) * rtiles + 1;
 }
 
 size_t Pipeline::pinned_bytes() const { return off_bounds() + 8 * (size_t)(K + 1) + 64; }
@@ -251,8 +256,8 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, st));
   uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
   for (int ps = 0; ps < passes; ++ps) {
-    NL_TRY(launch_radix_pass(kin, vin, kout, vout, n, kRadixBits * ps, b.rcnt, b.roff,
-                             b.rbounds, rstatus + rstiles * ps, tickets + ps, st));
+    NL_TRY(launch_radix_pass(kin, vin, kout, vout, n, digit_bits * ps, digit_bits, b.rcnt,
+                             b.roff, b.rbounds, rstatus + rstiles * ps, tickets + ps, st));
     std::swap(kin, kout);
     std::swap(vin, vout);
   }

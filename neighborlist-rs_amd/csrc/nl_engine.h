@@ -107,7 +107,7 @@ class Pipeline {
 
   int64_t n = 0, n_items = 0, bin_total = 0, scan_len = 0, rtiles = 0, stiles = 0,
           rstiles = 0, z_words = 0;
-  int S = 0, K = 0, passes = 0;
+  int S = 0, K = 0, passes = 0, digit_bits = 8;
   Tuning tune;
   Params prm{};
   bool pos_device = false, global_index = true, disjoint = false, per_sys = false;
@@ -118,8 +118,9 @@ class Pipeline {
   std::vector<int64_t> starts, item_base;
 
   // radix scratch block: counts (u32), then offsets and bounds (i64)
-  size_t roff_at() const { return (4 * 256 * (size_t)rtiles + 7) & ~size_t(7); }
-  size_t radix_bytes() const { return roff_at() + 8 * (256 * (size_t)rtiles + 3); }
+  size_t rdigits() const { return (size_t)1 << digit_bits; }
+  size_t roff_at() const { return (4 * rdigits() * (size_t)rtiles + 7) & ~size_t(7); }
+  size_t radix_bytes() const { return roff_at() + 8 * (rdigits() * (size_t)rtiles + 3); }
   // layout of the descriptor block (pinned staging and device alike)
   size_t off_start() const { return sizeof(SysDesc) * S; }
   size_t off_items() const { return off_start() + 8 * (size_t)(S + 1); }

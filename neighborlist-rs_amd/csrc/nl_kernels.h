@@ -45,11 +45,12 @@ hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const doub
                        const Params& p, hipStream_t st);
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
                       int64_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
-// one stable 8-bit pass: cnt [256 * tiles] u32, off [256 * tiles + 1] i64,
-// bounds [2] i64 and a zeroed scan status [radix_tiles(256 * tiles)] + ticket
+// one stable pass over digit bits [shift, shift + bits), bits <= 12:
+// cnt [2^bits * tiles] u32, off [2^bits * tiles + 1] i64, bounds [2] i64 and
+// a zeroed scan status [radix_tiles(2^bits * tiles)] + ticket
 hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                             uint32_t* vout, int64_t n, int shift, uint32_t* cnt, int64_t* off,
-                             int64_t* bounds, uint64_t* status, uint32_t* ticket,
+                             uint32_t* vout, int64_t n, int shift, int bits, uint32_t* cnt,
+                             int64_t* off, int64_t* bounds, uint64_t* status, uint32_t* ticket,
                              hipStream_t st);
 hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const double* pos,
                            const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,

@@ -310,69 +310,90 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ pos,
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, one 8-bit digit per pass, reduce-then-scan: per
-// tile (2048 keys) digit counts, one exclusive scan over (digit, tile) in
-// digit-major order (the scan kernel below), then a stable scatter.  No
+// Stable LSD radix sort, reduce-then-scan, digits of `bits` <= 12 bits:
+// per tile (2048 keys) digit counts, one exclusive scan over (digit, tile)
+// in digit-major order (the scan kernel below), then a stable scatter.  No
 // block waits on another, so a pass costs three short launches instead of a
 // chain of look-backs across tiles.
 //
 // Digit counts of one tile: cnt[digit * tiles + tile].
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ kin, int64_t n,
-                                                    int shift, int64_t tiles,
+                                                    int shift, int bits, int64_t tiles,
                                                     uint32_t* __restrict__ cnt) {
-  __shared__ uint32_t h[256];
+  __shared__ uint32_t h[kRadixMaxDigits];
   const int tid = threadIdx.x;
-  h[tid] = 0u;
+  const int nd = 1 << bits;
+  const uint32_t mask = (uint32_t)nd - 1u;
+  for (int d = tid; d < nd; d += 256) h[d] = 0u;
   __syncthreads();
   const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int64_t e = t0 + r * 256 + tid;
-    if (e < n) atomicAdd(&h[(kin[e] >> shift) & 255u], 1u);
+    if (e < n) atomicAdd(&h[(kin[e] >> shift) & mask], 1u);
   }
   __syncthreads();
-  cnt[(int64_t)tid * tiles + blockIdx.x] = h[tid];
+  for (int d = tid; d < nd; d += 256) cnt[(int64_t)d * tiles + blockIdx.x] = h[d];
 }
 
-// Ranks the keys of one tile (2048, 256 threads) by digit in element order
-// (8 rounds of 256, waves in order, lanes in order), so equal digits keep
-// their input order: lp[r] = s_run[digit] on entry + rank within the tile.
-// s_run[d] advances by the tile's count of digit d; s_cnt is zero on entry
-// and on exit.
+// Ranks the keys of one tile (2048 keys, 256 threads) by digit, stably.
+// Wave w owns the tile's keys [512 w, 512 w + 512) in 8 rounds of 64; it
+// ranks them with ballots and wave-private digit counters in LDS (the lanes
+// of one wave read a counter before its group leader advances it, in
+// instruction order, so no barrier is needed).  Then one pass over the
+// four waves' counters turns wave ranks into tile ranks.  Element order is
+// wave-major, rounds in order, lanes in order, so equal digits keep their
+// input order.  On return lp[r] = s_run[digit] on entry + rank within the
+// tile, and s_run[d] has advanced by the tile's count of digit d.
+// Element of thread (w, lane) in round r: t0 + 512 w + 64 r + lane.
 __device__ __forceinline__ void radix_rank_tile(const uint32_t* __restrict__ kin,
                                                 const uint32_t* __restrict__ vin, int64_t n,
-                                                int shift, int64_t t0, uint32_t* s_run,
-                                                uint32_t (*s_cnt)[256], uint32_t* kk,
+                                                int shift, int bits, int64_t t0, uint32_t* s_run,
+                                                uint32_t (*s_cnt)[kRadixMaxDigits], uint32_t* kk,
                                                 uint32_t* vv, uint32_t* lp) {
-  const int tid = threadIdx.x, w = tid >> 6;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int nd = 1 << bits;
+  const uint32_t mask = (uint32_t)nd - 1u;
+  uint32_t* wc = s_cnt[w];  // this wave's counters, zero on entry
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    const int64_t e = t0 + r * 256 + tid;
+    const int64_t e = t0 + 512 * w + 64 * r + lane;
     const bool valid = e < n;
     const uint32_t key = valid ? kin[e] : 0u;
-    const uint32_t dig = (key >> shift) & 255u;
+    const uint32_t dig = (key >> shift) & mask;
     kk[r] = key;
     vv[r] = valid ? vin[e] : 0u;
     uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < bits; ++b) {
       const bool bit = (dig >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
     }
     const uint32_t rank = lanes_below(peers);
-    if (valid && rank == 0) s_cnt[w][dig] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pre = s_run[dig];
-      for (int i = 0; i < w; ++i) pre += s_cnt[i][dig];
-      lp[r] = pre + rank;
-    }
-    __syncthreads();
-    s_run[tid] += s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
-    s_cnt[0][tid] = s_cnt[1][tid] = s_cnt[2][tid] = s_cnt[3][tid] = 0;
-    __syncthreads();
+    const uint32_t before = valid ? wc[dig] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wc[dig] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    lp[r] = before + rank;
   }
+  __syncthreads();
+  // wave prefix per digit
+  for (int d = tid; d < nd; d += 256) {
+    const uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
+    const uint32_t run = s_run[d];
+    s_cnt[0][d] = run;
+    s_cnt[1][d] = run + c0;
+    s_cnt[2][d] = run + c0 + c1;
+    s_cnt[3][d] = run + c0 + c1 + c2;
+    s_run[d] = run + c0 + c1 + c2 + c3;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) lp[r] += wc[(kk[r] >> shift) & mask];
+  __syncthreads();
+  for (int d = tid; d < nd; d += 256)
+    s_cnt[0][d] = s_cnt[1][d] = s_cnt[2][d] = s_cnt[3][d] = 0u;
+  __syncthreads();
 }
 
 // Scatter of one tile: off[digit * tiles + tile] is where the tile's keys
@@ -381,23 +402,25 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
                                                        const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout, int64_t n,
-                                                       int shift, int64_t tiles,
+                                                       int shift, int bits, int64_t tiles,
                                                        const int64_t* __restrict__ off) {
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_cnt[4][256];
-  const int tid = threadIdx.x;
-  s_run[tid] = 0;
-  for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
+  __shared__ uint32_t s_run[kRadixMaxDigits];
+  __shared__ uint32_t s_cnt[4][kRadixMaxDigits];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int nd = 1 << bits;
+  const uint32_t mask = (uint32_t)nd - 1u;
+  for (int d = tid; d < nd; d += 256) s_run[d] = s_cnt[0][d] = s_cnt[1][d] = s_cnt[2][d] =
+      s_cnt[3][d] = 0u;
   __syncthreads();
   const int64_t tile = blockIdx.x;
   const int64_t t0 = tile * kRadixTile;
   uint32_t kk[8], vv[8], lp[8];
-  radix_rank_tile(kin, vin, n, shift, t0, s_run, s_cnt, kk, vv, lp);
+  radix_rank_tile(kin, vin, n, shift, bits, t0, s_run, s_cnt, kk, vv, lp);
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    const int64_t e = t0 + r * 256 + tid;
+    const int64_t e = t0 + 512 * w + 64 * r + lane;
     if (e < n) {
-      const uint32_t dig = (kk[r] >> shift) & 255u;
+      const uint32_t dig = (kk[r] >> shift) & mask;
       const int64_t dst = off[(int64_t)dig * tiles + tile] + lp[r];
       kout[dst] = kk[r];
       vout[dst] = vv[r];
@@ -405,33 +428,47 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   }
 }
 
-// A whole pass in one block for small inputs (a few tiles): digit totals,
-// their exclusive scan, then the tiles ranked in order with running digit
-// positions.  One launch instead of three.
+// A whole pass in one block for small inputs (one or two tiles): digit
+// totals, their exclusive scan, then the tiles ranked in order with running
+// digit positions.  One launch instead of three.
 __global__ __launch_bounds__(256) void k_radix_small(const uint32_t* __restrict__ kin,
                                                      const uint32_t* __restrict__ vin,
                                                      uint32_t* __restrict__ kout,
                                                      uint32_t* __restrict__ vout, int64_t n,
-                                                     int shift) {
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_cnt[4][256];
+                                                     int shift, int bits) {
+  __shared__ uint32_t s_run[kRadixMaxDigits];
+  __shared__ uint32_t s_cnt[4][kRadixMaxDigits];
   __shared__ uint32_t s_tmp[4];
   const int tid = threadIdx.x;
-  s_run[tid] = 0;
-  for (int i = 0; i < 4; ++i) s_cnt[i][tid] = 0;
+  const int nd = 1 << bits;
+  const uint32_t mask = (uint32_t)nd - 1u;
+  for (int d = tid; d < nd; d += 256) s_run[d] = s_cnt[0][d] = s_cnt[1][d] = s_cnt[2][d] =
+      s_cnt[3][d] = 0u;
   __syncthreads();
-  for (int64_t e = tid; e < n; e += 256) atomicAdd(&s_run[(kin[e] >> shift) & 255u], 1u);
+  for (int64_t e = tid; e < n; e += 256) atomicAdd(&s_run[(kin[e] >> shift) & mask], 1u);
   __syncthreads();
-  const uint32_t tot = s_run[tid];
-  const uint32_t base = block_excl_scan_256(tot, s_tmp, nullptr);
-  s_run[tid] = base;
+  // exclusive scan of the digit totals: each thread owns a run of digits
+  const int per = nd / 256 > 0 ? nd / 256 : 1;
+  const int d0 = tid * per;
+  uint32_t local = 0;
+  for (int i = 0; i < per; ++i)
+    if (d0 + i < nd) local += s_run[d0 + i];
+  uint32_t acc = block_excl_scan_256(local, s_tmp, nullptr);
+  for (int i = 0; i < per; ++i) {
+    if (d0 + i < nd) {
+      const uint32_t c = s_run[d0 + i];
+      s_run[d0 + i] = acc;
+      acc += c;
+    }
+  }
   __syncthreads();
+  const int w = tid >> 6, lane = tid & 63;
   for (int64_t t0 = 0; t0 < n; t0 += kRadixTile) {
     uint32_t kk[8], vv[8], lp[8];
-    radix_rank_tile(kin, vin, n, shift, t0, s_run, s_cnt, kk, vv, lp);
+    radix_rank_tile(kin, vin, n, shift, bits, t0, s_run, s_cnt, kk, vv, lp);
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int64_t e = t0 + r * 256 + tid;
+      const int64_t e = t0 + 512 * w + 64 * r + lane;
       if (e < n) {
         kout[lp[r]] = kk[r];
         vout[lp[r]] = vv[r];
@@ -1235,23 +1272,25 @@ hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_
 }
 
 hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                             uint32_t* vout, int64_t n, int shift, uint32_t* cnt, int64_t* off,
-                             int64_t* bounds, uint64_t* status, uint32_t* ticket,
+                             uint32_t* vout, int64_t n, int shift, int bits, uint32_t* cnt,
+                             int64_t* off, int64_t* bounds, uint64_t* status, uint32_t* ticket,
                              hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  if (bits < 1 || bits > kRadixMaxBits) return hipErrorInvalidValue;
   const int64_t tiles = radix_tiles(n);
   if (tiles <= 2) {  // one block ranks the tiles in order: one launch
     // (one block is slower than three launches from about 3 tiles on)
-    hipLaunchKernelGGL(k_radix_small, dim3(1), dim3(256), 0, st, kin, vin, kout, vout, n, shift);
+    hipLaunchKernelGGL(k_radix_small, dim3(1), dim3(256), 0, st, kin, vin, kout, vout, n, shift,
+                       bits);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(256), 0, st, kin, n, shift, tiles,
-                     cnt);
-  const hipError_t e = launch_scan(cnt, off, 256 * tiles, 256 * tiles, 1, bounds, status, ticket,
-                                   st);
+  const int64_t len = ((int64_t)1 << bits) * tiles;
+  hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(256), 0, st, kin, n, shift, bits,
+                     tiles, cnt);
+  const hipError_t e = launch_scan(cnt, off, len, len, 1, bounds, status, ticket, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(256), 0, st, kin, vin, kout,
-                     vout, n, shift, tiles, (const int64_t*)off);
+                     vout, n, shift, bits, tiles, (const int64_t*)off);
   return hipGetLastError();
 }
 
