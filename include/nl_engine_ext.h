@@ -96,9 +96,9 @@ void nl_plan_free(nl_plan* p);
 /* ---- serving: continuous batching -----------------------------------------
  * No counterpart in the reference (its API is synchronous, one call per
  * request); this is the serving form of nl_search_single with one cutoff.
- * Requests are submitted from any thread and return an id at once.  A worker
- * thread keeps up to `slots` ragged batches in flight, each on its own HIP
- * stream with its own buffers; a free slot takes the queued requests in
+ * Requests are submitted from any thread and return an id at once.  Up to
+ * `slots` ragged batches are in flight, each on its own HIP stream with its
+ * own buffers and host thread; a free slot takes the queued requests in
  * arrival order, up to max_atoms atoms and max_systems systems per batch,
  * while the estimated edges of all batches on the device stay within
  * edge_budget (a request alone is always admitted).  Each request keeps its
@@ -130,7 +130,7 @@ int nl_server_release(nl_server* s, int64_t id);
  * (their fill pass ran a second time into larger buffers) */
 int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t* max_inflight,
                     int64_t* max_batch, int64_t* refills);
-/* waits for every submitted request, stops the worker, frees the buffers */
+/* waits for every submitted request, stops the slot threads, frees the buffers */
 void nl_server_free(nl_server* s);
 
 /* Host-side cell helpers with the reference's arithmetic (no GPU needed):

@@ -1,8 +1,8 @@
 // nl_server.cpp -- serving: continuous batching of single-system requests.
 //
 // Requests (positions, optional cell, one cutoff each) arrive at any time.
-// A worker thread keeps up to `slots` batches in flight, each on its own HIP
-// stream with its own device buffers.  Whenever a slot is free it admits the
+// Up to `slots` batches are in flight, each on its own HIP stream with its
+// own device buffers.  Whenever a slot is free it admits the
 // queued requests in arrival order into one ragged batch (per-system cutoffs,
 // local indices), bounded by atoms and systems per batch and by an edge
 // budget: every request carries an estimate of its edge count, and the sum
@@ -15,9 +15,11 @@
 // and shifts of all its requests), and each request of the batch holds a
 // reference to the block until the caller takes (copies out) or releases
 // it.  Blocks are recycled through a pool, so the steady state allocates
-// nothing.  Per slot the worker drives a small state machine on events and
-// never blocks on one slot while others run:
-//   launched -> counts known -> [refill if the estimate was short] -> done
+// nothing.  Each slot has a host thread of its own, so the host work of one
+// batch (staging, preparation, launches) overlaps the device work and host
+// work of the others:
+//   admit (FIFO, budget) -> launch -> counts known -> [refill if the
+//   estimate was short] -> deliver
 #include "nl_server.h"
 
 #include <hip/hip_runtime.h>
@@ -107,6 +109,7 @@ struct Server::Slot {
   OutBlock* blk = nullptr;
   int64_t est_used = 0;
   std::vector<SrvReq*> reqs;
+  std::thread th;
 };
 
 static bool grow_pinned(void** p, size_t* cap, size_t bytes) {
@@ -149,7 +152,7 @@ Server::Server(Engine* eng, int slots, int64_t max_atoms, int64_t max_systems,
     s->caps.assign(kNumBuffers, 0);
     slots_.push_back(s);
   }
-  th_ = std::thread([this] { loop(); });
+  for (Slot* s : slots_) s->th = std::thread([this, s] { slot_loop(*s); });
 }
 
 Server::~Server() {
@@ -158,7 +161,8 @@ Server::~Server() {
     stop_ = true;
   }
   cv_.notify_all();
-  if (th_.joinable()) th_.join();
+  for (Slot* s : slots_)
+    if (s->th.joinable()) s->th.join();
   (void)hipSetDevice(eng_->device());
   for (Slot* s : slots_) (void)hipStreamSynchronize(s->st);
   for (Slot* s : slots_) {
@@ -341,7 +345,9 @@ void Server::fail_slot(Slot& s, const std::string& msg) {
   s.est_used = 0;
   s.reqs.clear();
   s.state = 0;
+  --busy_;
   done_cv_.notify_all();
+  cv_.notify_all();
 }
 
 // An output block for at least `cap` edges: the smallest pooled block that
@@ -379,7 +385,7 @@ OutBlock* Server::get_block(int64_t cap) {
   return b;
 }
 
-// Launch one batch on a free slot (called by the worker, queue lock not held).
+// Launch one batch on a slot (called by the slot's thread, queue lock not held).
 bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   Request req;
   int64_t n = 0;
@@ -461,12 +467,13 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
 }
 
 // Counts are known: deliver, or first refill into a larger block if the
-// estimate was short (the first fill pass then wrote nothing).
-void Server::on_counted(Slot& s) {
+// estimate was short (the first fill pass then wrote nothing).  Returns 0
+// when delivered, 1 when a refill is on the stream, -1 on failure.
+int Server::on_counted(Slot& s) {
   const int64_t total = s.P.host_bounds(s.pin)[1];
   if (total <= s.blk->cap) {
     deliver(s, total);
-    return;
+    return 0;
   }
   {
     std::lock_guard<std::mutex> g(mu_);
@@ -476,16 +483,17 @@ void Server::on_counted(Slot& s) {
   s.blk = get_block(total + total / 8 + 4096);
   if (!s.blk) {
     fail_slot(s, "out of device memory for the batch's edge lists");
-    return;
+    return -1;
   }
   hipError_t e =
       s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), nullptr, s.blk->cap, s.st);
   if (e == hipSuccess) e = hipEventRecord(s.ev, s.st);
   if (e != hipSuccess) {
     fail_slot(s, std::string("HIP error in the server: ") + hipGetErrorString(e));
-    return;
+    return -1;
   }
   s.state = 2;
+  return 1;
 }
 
 void Server::deliver(Slot& s, int64_t total) {
@@ -507,74 +515,66 @@ void Server::deliver(Slot& s, int64_t total) {
   s.est_used = 0;
   s.reqs.clear();
   s.state = 0;
+  --busy_;
   done_cv_.notify_all();
+  cv_.notify_all();
 }
 
-void Server::loop() {
+// The head of the queue may start now (mu_ held): it fits the edge budget
+// next to the batches in flight, or nothing is in flight at all.
+bool Server::admissible_locked() const {
+  return !queue_.empty() && (busy_ == 0 || inflight_est_ + queue_.front()->est <= budget_);
+}
+
+void Server::slot_loop(Slot& s) {
   (void)hipSetDevice(eng_->device());
   while (true) {
-    bool progress = false;
-    // 1. advance slots whose device work finished
-    for (Slot* s : slots_) {
-      if (s->state == 0 || hipEventQuery(s->ev) != hipSuccess) continue;
-      progress = true;
-      if (s->state == 1)
-        on_counted(*s);
-      else
-        deliver(*s, s->P.host_bounds(s->pin)[1]);
-    }
-    // 2. admit queued requests into free slots
-    for (Slot* s : slots_) {
-      if (s->state != 0) continue;
-      std::vector<SrvReq*> batch;
-      int64_t est = 0;
-      {
-        std::lock_guard<std::mutex> g(mu_);
-        if (queue_.empty()) break;
-        int64_t atoms = 0;
-        int busy = 0;
-        for (Slot* o : slots_) busy += o->state != 0;
-        while (!queue_.empty()) {
-          SrvReq* r = queue_.front();
-          const int64_t na = (int64_t)(r->pos.size() / 3);
-          const bool first = batch.empty();
-          if (!first && (atoms + na > max_atoms_ || (int64_t)batch.size() >= max_systems_))
-            break;
-          // edge-budget admission; a lone request is admitted when nothing
-          // else is on the device, so oversized requests cannot starve
-          if (inflight_est_ + est + r->est > budget_ && !(first && busy == 0)) break;
-          batch.push_back(r);
-          atoms += na;
-          est += r->est;
-          queue_.pop_front();
-        }
-        if (batch.empty()) break;
-        inflight_est_ += est;
-        for (SrvReq* r : batch) {
-          r->state = 1;
-          r->batch_size = (int)batch.size();
-        }
-        ++batches_;
-        max_batch_ = std::max<int64_t>(max_batch_, (int64_t)batch.size());
+    std::vector<SrvReq*> batch;
+    int64_t est = 0;
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_.wait(g, [&] { return admissible_locked() || (stop_ && queue_.empty()); });
+      if (queue_.empty()) break;  // stopping, nothing left
+      int64_t atoms = 0;
+      while (!queue_.empty()) {
+        SrvReq* r = queue_.front();
+        const int64_t na = (int64_t)(r->pos.size() / 3);
+        const bool first = batch.empty();
+        if (!first && (atoms + na > max_atoms_ || (int64_t)batch.size() >= max_systems_))
+          break;
+        // edge-budget admission; a lone request is admitted when nothing
+        // else is on the device, so oversized requests cannot starve
+        if (inflight_est_ + est + r->est > budget_ && !(first && busy_ == 0)) break;
+        batch.push_back(r);
+        atoms += na;
+        est += r->est;
+        queue_.pop_front();
       }
-      s->est_used = est;
-      if (launch(*s, batch)) progress = true;
-      int busy = 0;
-      for (Slot* o : slots_) busy += o->state != 0;
-      std::lock_guard<std::mutex> g(mu_);
-      max_inflight_ = std::max<int64_t>(max_inflight_, busy);
+      inflight_est_ += est;
+      ++busy_;
+      max_inflight_ = std::max<int64_t>(max_inflight_, busy_);
+      for (SrvReq* r : batch) {
+        r->state = 1;
+        r->batch_size = (int)batch.size();
+      }
+      ++batches_;
+      max_batch_ = std::max<int64_t>(max_batch_, (int64_t)batch.size());
+      if (!queue_.empty()) cv_.notify_all();  // another slot may take the rest
     }
-    // 3. idle: wait for new requests, or poll the device briefly
-    bool any_busy = false;
-    for (Slot* s : slots_) any_busy |= s->state != 0;
-    std::unique_lock<std::mutex> g(mu_);
-    if (stop_ && !any_busy && queue_.empty()) break;
-    if (!progress) {
-      if (any_busy)
-        cv_.wait_for(g, std::chrono::microseconds(20));
-      else if (queue_.empty() && !stop_)
-        cv_.wait_for(g, std::chrono::milliseconds(5));
+    s.est_used = est;
+    if (!launch(s, batch)) continue;  // failed and accounted for
+    hipError_t e = hipEventSynchronize(s.ev);
+    if (e != hipSuccess) {
+      fail_slot(s, std::string("HIP error in the server: ") + hipGetErrorString(e));
+      continue;
     }
+    if (on_counted(s) != 1) continue;  // delivered (or failed)
+    e = hipEventSynchronize(s.ev);
+    if (e != hipSuccess) {
+      fail_slot(s, std::string("HIP error in the server: ") + hipGetErrorString(e));
+      continue;
+    }
+    deliver(s, s.P.host_bounds(s.pin)[1]);
   }
 }
 

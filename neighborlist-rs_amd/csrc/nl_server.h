@@ -35,9 +35,10 @@ class Server {
 
  private:
   struct Slot;
-  void loop();
+  void slot_loop(Slot& s);
+  bool admissible_locked() const;
   bool launch(Slot& s, std::vector<SrvReq*>& batch);
-  void on_counted(Slot& s);
+  int on_counted(Slot& s);
   void deliver(Slot& s, int64_t total);
   void fail_slot(Slot& s, const std::string& msg);
   OutBlock* get_block(int64_t cap);
@@ -47,7 +48,6 @@ class Server {
   Engine* eng_;
   int64_t max_atoms_, max_systems_, budget_;
   std::vector<Slot*> slots_;
-  std::thread th_;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
   std::deque<SrvReq*> queue_;
@@ -55,7 +55,7 @@ class Server {
   std::vector<OutBlock*> pool_;  // free output blocks
   bool stop_ = false;
   int64_t next_id_ = 1, completed_ = 0, batches_ = 0, max_inflight_ = 0, max_batch_ = 0,
-          inflight_est_ = 0, refills_ = 0;
+          inflight_est_ = 0, refills_ = 0, busy_ = 0;
 };
 
 }  // namespace nl
