@@ -37,6 +37,35 @@ int nl_search_systems(const double* positions, size_t n, const int64_t* sys_size
                       int multi, int global_index, int want_ranges, int want_dist,
                       nl_result** out);
 
+/* The general form of the two above, for callers that keep data on the GPU
+ * (a model's step, a serving loop).  options is a bit set of the NL_* flags
+ * below.  stream is the caller's HIP stream (hipStream_t; NULL = the null
+ * stream): the search reads device-resident positions after the work
+ * already queued there, and output blocks released through DLPack views
+ * are handed out again only after the work queued there before the release.
+ *
+ * NL_FULL gives full (directed) lists: every pair of the half list in both
+ * directions, (i, j, S) at an even position e and (j, i, -S) at e + 1; a
+ * self image (i, i, S) appears once per S (the half list already holds both
+ * S and -S).  Per-system ranges and distances cover the full lists. */
+#define NL_DEVICE_POSITIONS 1u /* positions is a device pointer */
+#define NL_LOCAL_INDEX 2u      /* indices local to each system (else offset by its start) */
+#define NL_RANGES 4u           /* per-system edge ranges (nl_result_system_ranges) */
+#define NL_DIST 8u             /* FP64 |r_ij| per edge (nl_result_copy_dist) */
+#define NL_FULL 16u            /* full (directed) lists */
+int nl_search_ex(const double* positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                 const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                 const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
+                 unsigned options, void* stream, nl_result** out);
+
+/* A zero-copy DLPack view (DLManagedTensor*, ROCm device) of one array of
+ * list `list`: what = 0 edge_index (2, M) int64, 1 shift (M, 3) int32,
+ * 2 distance (M,) float64.  The view shares ownership of the result: the
+ * lists stay valid until nl_result_free and the deleter of every view have
+ * run, whichever comes last.  Work on the arrays belongs on the stream given
+ * to nl_search_ex; the view's deleter fences that stream. */
+int nl_result_dlpack(const nl_result* r, size_t list, int what, void** managed);
+
 /* ranges[(s * n_lists + list) * 2 + {0,1}] = [first, end) edge of system s
  * within list `list` (input cutoff order); mode[s] = 0 brute force on raw
  * positions, 1 brute force minimum image, 2 cell list.  Either may be NULL. */

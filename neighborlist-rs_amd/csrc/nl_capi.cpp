@@ -3,6 +3,7 @@
 // reference's dispatch layer; all searching happens in the engine.
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -16,7 +17,7 @@ using nl::Request;
 using nl::Result;
 
 struct nl_result {
-  Result* r;
+  std::shared_ptr<Result> r;  // shared with exported (DLPack) views
 };
 
 struct nl_plan {
@@ -32,7 +33,7 @@ struct nl_server {
 namespace {
 
 int wrap_result(Result* r, nl_result** out) {
-  *out = new nl_result{r};
+  *out = new nl_result{std::shared_ptr<Result>(r)};
   return 0;
 }
 
@@ -217,23 +218,38 @@ int nl_search_batch_multi(const double* positions, size_t n, const int32_t* batc
   return run(req, out);
 }
 
-static int systems_common(bool device, const double* positions, size_t n,
-                          const int64_t* sys_sizes, size_t n_sys, const double* cells,
-                          const uint8_t* has_cell, const uint8_t* pbc, const double* cutoffs,
-                          size_t n_cutoffs, int disjoint, int multi, int global_index,
-                          int want_ranges, int want_dist, nl_result** out) {
+static int search_opts(const double* positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                       const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                       const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
+                       unsigned options, bool has_stream, void* stream, nl_result** out) {
   if (check_cutoffs(cutoffs, n_cutoffs, "All cutoffs must be positive and finite")) return 1;
   if (n == 0 || n_cutoffs == 0) return empty_result((int)n_cutoffs, out);
   Request req;
   if (sizes_request(positions, n, sys_sizes, n_sys, cells, has_cell, pbc, req)) return 1;
-  req.pos_device = device;
+  req.pos_device = (options & NL_DEVICE_POSITIONS) != 0;
   req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
   req.multi = multi != 0;
   req.disjoint = disjoint != 0;
-  req.global_index = global_index != 0;
-  req.want_ranges = want_ranges != 0;
-  req.want_dist = want_dist != 0;
+  req.global_index = (options & NL_LOCAL_INDEX) == 0;
+  req.want_ranges = (options & NL_RANGES) != 0;
+  req.want_dist = (options & NL_DIST) != 0;
+  req.full = (options & NL_FULL) != 0;
+  req.has_ustream = has_stream;
+  req.ustream = (hipStream_t)stream;
   return run(req, out);
+}
+
+int nl_search_ex(const double* positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                 const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                 const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
+                 unsigned options, void* stream, nl_result** out) {
+  return search_opts(positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs, n_cutoffs,
+                     disjoint, multi, options, true, stream, out);
+}
+
+static unsigned legacy_options(bool device, int global_index, int want_ranges, int want_dist) {
+  return (device ? NL_DEVICE_POSITIONS : 0u) | (global_index ? 0u : NL_LOCAL_INDEX) |
+         (want_ranges ? NL_RANGES : 0u) | (want_dist ? NL_DIST : 0u);
 }
 
 int nl_search_batch_device(const double* d_positions, size_t n, const int64_t* sys_sizes,
@@ -241,8 +257,9 @@ int nl_search_batch_device(const double* d_positions, size_t n, const int64_t* s
                            const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs,
                            int disjoint, int multi, int global_index, int want_ranges,
                            int want_dist, nl_result** out) {
-  return systems_common(true, d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs,
-                        n_cutoffs, disjoint, multi, global_index, want_ranges, want_dist, out);
+  return search_opts(d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs, n_cutoffs,
+                     disjoint, multi, legacy_options(true, global_index, want_ranges, want_dist),
+                     false, nullptr, out);
 }
 
 int nl_search_systems(const double* positions, size_t n, const int64_t* sys_sizes,
@@ -250,8 +267,9 @@ int nl_search_systems(const double* positions, size_t n, const int64_t* sys_size
                       const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs, int disjoint,
                       int multi, int global_index, int want_ranges, int want_dist,
                       nl_result** out) {
-  return systems_common(false, positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs,
-                        n_cutoffs, disjoint, multi, global_index, want_ranges, want_dist, out);
+  return search_opts(positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs, n_cutoffs,
+                     disjoint, multi, legacy_options(false, global_index, want_ranges, want_dist),
+                     false, nullptr, out);
 }
 
 size_t nl_result_num_lists(const nl_result* r) { return r ? (size_t)r->r->k : 0; }
@@ -267,7 +285,7 @@ int nl_result_copy(const nl_result* r, size_t list, int64_t* edge_i, int64_t* ed
     nl::set_error("invalid result or list index");
     return 1;
   }
-  const Result* R = r->r;
+  const Result* R = r->r.get();
   const int64_t m = R->len[list];
   if (m == 0) return 0;
   const int64_t b = R->base[list];
@@ -294,7 +312,7 @@ int nl_result_copy_dist(const nl_result* r, size_t list, double* dist) {
     nl::set_error("no distances recorded for this result");
     return 1;
   }
-  const Result* R = r->r;
+  const Result* R = r->r.get();
   const int64_t m = R->len[list];
   if (m == 0) return 0;
   hipStream_t st = R->eng->stream();
@@ -314,7 +332,7 @@ int nl_result_device(const nl_result* r, size_t list, const int64_t** edge_index
     nl::set_error("invalid result or list index");
     return 1;
   }
-  const Result* R = r->r;
+  const Result* R = r->r.get();
   const int64_t b = R->base[list];
   if (edge_index) *edge_index = R->edges ? (const int64_t*)R->edges + 2 * b : nullptr;
   if (shifts) *shifts = R->shifts ? (const int32_t*)R->shifts + 3 * b : nullptr;
@@ -326,7 +344,7 @@ int nl_result_system_ranges(const nl_result* r, int64_t* ranges, int32_t* modes)
     nl::set_error("invalid result");
     return 1;
   }
-  const Result* R = r->r;
+  const Result* R = r->r.get();
   if (R->ranges.empty() && R->total > 0) {
     nl::set_error("system ranges were not requested for this result");
     return 1;
@@ -337,10 +355,102 @@ int nl_result_system_ranges(const nl_result* r, int64_t* ranges, int32_t* modes)
   return 0;
 }
 
-void nl_result_free(nl_result* r) {
-  if (!r) return;
-  delete r->r;
-  delete r;
+void nl_result_free(nl_result* r) { delete r; }
+
+// ---- DLPack export ---------------------------------------------------------
+// The DLPack ABI (DLManagedTensor of dlpack.h, v0.8): laid out here field for
+// field, as every producer does, since consumers bind to exactly this layout.
+namespace {
+struct DLDevice_ {
+  int32_t device_type;  // kDLROCM = 10
+  int32_t device_id;
+};
+struct DLDataType_ {
+  uint8_t code;  // kDLInt = 0, kDLFloat = 2
+  uint8_t bits;
+  uint16_t lanes;
+};
+struct DLTensor_ {
+  void* data;
+  DLDevice_ device;
+  int32_t ndim;
+  DLDataType_ dtype;
+  int64_t* shape;
+  int64_t* strides;
+  uint64_t byte_offset;
+};
+struct DLManagedTensor_ {
+  DLTensor_ dl_tensor;
+  void* manager_ctx;
+  void (*deleter)(DLManagedTensor_*);
+};
+struct DlView {
+  std::shared_ptr<Result> r;
+  int64_t shape[2];
+  int64_t strides[2];
+  DLManagedTensor_ m;
+};
+void dl_release(DLManagedTensor_* m) {
+  DlView* v = (DlView*)m->manager_ctx;
+  v->r->fence_on_consumer();
+  delete v;
+}
+}  // namespace
+
+int nl_result_dlpack(const nl_result* r, size_t list, int what, void** out) {
+  if (!out) {
+    nl::set_error("output pointer is NULL");
+    return 1;
+  }
+  *out = nullptr;
+  if (!r || list >= (size_t)r->r->k || what < 0 || what > 2) {
+    nl::set_error("invalid result, list index or array selector");
+    return 1;
+  }
+  const Result* R = r->r.get();
+  if (!R->eng) {
+    nl::set_error("an empty result has no device arrays");
+    return 1;
+  }
+  if (what == 2 && !R->dist) {
+    nl::set_error("no distances recorded for this result");
+    return 1;
+  }
+  const int64_t m = R->len[list], b = R->base[list];
+  DlView* v = new DlView();
+  v->r = r->r;
+  DLTensor_& t = v->m.dl_tensor;
+  t.device = DLDevice_{10, R->eng->device()};
+  t.byte_offset = 0;
+  t.shape = v->shape;
+  t.strides = v->strides;
+  if (what == 0) {  // edge_index (2, M) int64: sources, then targets
+    t.data = (int64_t*)R->edges + 2 * b;
+    t.ndim = 2;
+    t.dtype = DLDataType_{0, 64, 1};
+    v->shape[0] = 2;
+    v->shape[1] = m;
+    v->strides[0] = m;
+    v->strides[1] = 1;
+  } else if (what == 1) {  // shift (M, 3) int32
+    t.data = (int32_t*)R->shifts + 3 * b;
+    t.ndim = 2;
+    t.dtype = DLDataType_{0, 32, 1};
+    v->shape[0] = m;
+    v->shape[1] = 3;
+    v->strides[0] = 3;
+    v->strides[1] = 1;
+  } else {  // distance (M,) float64
+    t.data = (double*)R->dist + b;
+    t.ndim = 1;
+    t.dtype = DLDataType_{2, 64, 1};
+    v->shape[0] = m;
+    v->strides[0] = 1;
+  }
+  v->m.manager_ctx = v;
+  v->m.deleter = dl_release;
+  *out = &v->m;
+  return 0;
 }
 
 size_t nl_get_brute_force_threshold(void) { return nl::config().bf_threshold.load(); }

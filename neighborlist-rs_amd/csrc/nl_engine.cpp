@@ -22,11 +22,38 @@ static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
 
+Fence::~Fence() {
+  if (ev) (void)hipEventDestroy(ev);
+}
+
+// Called when an exported view of this result is dropped: (re)records the
+// fence on the consumer's stream, so the last drop covers every use queued
+// there before it.
+void Result::fence_on_consumer() {
+  if (!eng) return;
+  int old = -1;
+  (void)hipGetDevice(&old);
+  if (old != eng->device()) (void)hipSetDevice(eng->device());
+  {
+    std::lock_guard<std::mutex> g(fence_mu);
+    if (!fence) {
+      fence = std::make_shared<Fence>();
+      if (hipEventCreateWithFlags(&fence->ev, hipEventDisableTiming) != hipSuccess)
+        fence->ev = nullptr;
+    }
+    if (fence->ev && hipEventRecord(fence->ev, ustream) != hipSuccess) {
+      // no fence: wait here instead
+      (void)hipStreamSynchronize(ustream);
+    }
+  }
+  if (old >= 0 && old != eng->device()) (void)hipSetDevice(old);
+}
+
 Result::~Result() {
   if (!eng) return;
-  if (edges) eng->dev_release(edges, edges_bytes);
-  if (shifts) eng->dev_release(shifts, shifts_bytes);
-  if (dist) eng->dev_release(dist, dist_bytes);
+  if (edges) eng->dev_release(edges, edges_bytes, fence);
+  if (shifts) eng->dev_release(shifts, shifts_bytes, fence);
+  if (dist) eng->dev_release(dist, dist_bytes, fence);
 }
 
 static int bits_for(int64_t v) {
@@ -57,6 +84,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   host_pos = req.pos;
   global_index = req.global_index;
   disjoint = req.disjoint;
+  full = req.full;
   // cutoffs: stable ascending order (Rust sort_by is stable, search.rs:354-356),
   // squares in f64 (search.rs:357-360)
   order.resize(K);
@@ -241,6 +269,7 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.use_fp32 = P.tune.use_fp32;
   A.global_index = P.global_index;
   A.per_sys_rc = P.per_sys;
+  A.full = P.full ? 1 : 0;
   A.group = P.tune.group;
   A.parts = P.tune.parts;
   return A;
@@ -348,33 +377,50 @@ void* Engine::pinned(size_t bytes) {
   return pinned_;
 }
 
+// Pooled device blocks: the smallest one of at least `bytes` (at most
+// twice that), preferring blocks whose consumer fence has completed; a
+// block still fenced is waited for only when no other fits.
 void* Engine::dev_alloc(size_t bytes) {
   bytes = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
+  std::shared_ptr<Fence> wait;
+  void* got = nullptr;
   {
     std::lock_guard<std::mutex> g(pool_mu_);
-    auto it = pool_.lower_bound(bytes);
-    if (it != pool_.end() && it->first <= 2 * bytes) {
-      void* p = it->second;
-      pool_.erase(it);
-      return p;
+    auto pick = pool_.end();
+    for (auto it = pool_.lower_bound(bytes); it != pool_.end() && it->first <= 2 * bytes; ++it) {
+      const Fence* f = it->second.fence.get();
+      if (!f || !f->ev || hipEventQuery(f->ev) != hipErrorNotReady) {
+        pick = it;
+        break;
+      }
+      if (pick == pool_.end()) pick = it;
     }
+    if (pick != pool_.end()) {
+      got = pick->second.p;
+      wait = pick->second.fence;
+      pool_.erase(pick);
+    }
+  }
+  if (got) {
+    if (wait && wait->ev) (void)hipEventSynchronize(wait->ev);
+    return got;
   }
   void* p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) {
-    // release cached blocks and retry once
+    // release cached blocks (hipFree waits for the device) and retry once
     std::lock_guard<std::mutex> g(pool_mu_);
-    for (auto& kv : pool_) (void)hipFree(kv.second);
+    for (auto& kv : pool_) (void)hipFree(kv.second.p);
     pool_.clear();
     if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
   }
   return p;
 }
 
-void Engine::dev_release(void* p, size_t bytes) {
+void Engine::dev_release(void* p, size_t bytes, std::shared_ptr<Fence> fence) {
   if (!p) return;
   bytes = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
   std::lock_guard<std::mutex> g(pool_mu_);
-  pool_.emplace(bytes, p);
+  pool_.emplace(bytes, PoolBlock{p, std::move(fence)});
 }
 
 int Engine::alloc_outputs(Result* res, int64_t edges, bool dist) {
@@ -418,6 +464,8 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
   res->len.assign(K, 0);
   res->base.assign(K, 0);
   res->tune = t;
+  res->full = req.full;
+  res->ustream = req.has_ustream ? req.ustream : nullptr;
   if (req.n == 0 || req.sys.empty()) {
     *out = res.release();
     return 0;
@@ -449,6 +497,11 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
   P.stage_host(pin);
 
   hipStream_t st = stream_;
+  if (req.has_ustream) {
+    // device-resident inputs are read after the caller's queued work
+    NL_CHECK(hipEventRecord(ev_[5], req.ustream));
+    NL_CHECK(hipStreamWaitEvent(st, ev_[5], 0));
+  }
   NL_CHECK(P.enqueue_upload(B, pin, st));
   if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
   NL_CHECK(P.enqueue_build_count(B, st));
@@ -456,7 +509,7 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
 
   // Fill straight away when an estimate of the output size exists (same
   // request shape as last time); the kernel writes nothing if it is short.
-  const int64_t sig[4] = {P.n, S, K, (int64_t)req.multi * 2 + req.disjoint};
+  const int64_t sig[4] = {P.n, S, K, (int64_t)req.full * 4 + (int64_t)req.multi * 2 + req.disjoint};
   const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
   const int64_t cap = have_est ? last_total_ + last_total_ / 8 + 4096 : -1;
   auto launch_fill = [&](int64_t capacity) -> int {

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -50,6 +51,11 @@ struct Request {
   bool global_index = true;
   bool want_ranges = false;     // per-system edge ranges
   bool want_dist = false;       // FP64 |r_ij| per edge
+  bool full = false;            // full lists: (i, j, S) and (j, i, -S) for every pair
+  // the caller's stream: device-resident positions are read after the work
+  // already queued on it, and released outputs are reused only after it
+  bool has_ustream = false;
+  hipStream_t ustream = nullptr;
   int cell_div = 0;             // 0: tuned / engine default
   int use_fp32 = -1;            // -1: tuned / engine default
   int group = 0;                // 0: tuned / engine default
@@ -110,7 +116,7 @@ class Pipeline {
   int S = 0, K = 0, passes = 0, digit_bits = 8;
   Tuning tune;
   Params prm{};
-  bool pos_device = false, global_index = true, disjoint = false, per_sys = false;
+  bool pos_device = false, global_index = true, disjoint = false, per_sys = false, full = false;
   const double* host_pos = nullptr;
   std::vector<int> order;  // sorted rank -> input list
   std::vector<double> rc2; // sorted
@@ -130,6 +136,13 @@ class Pipeline {
 
 class Engine;
 
+// An event recorded on a consumer's stream when it released device memory:
+// the memory is handed out again only once the event has completed.
+struct Fence {
+  hipEvent_t ev = nullptr;
+  ~Fence();
+};
+
 // One request's output (the nl_result of the C ABI).
 struct Result {
   Engine* eng = nullptr;
@@ -146,6 +159,13 @@ struct Result {
   std::vector<int64_t> ranges; // [s][list(input order)][2] global edge ids (optional)
   std::vector<int32_t> modes;  // per system (optional, with ranges)
   Tuning tune;
+  bool full = false;
+  // consumer stream of exported views (DLPack): the output blocks go back
+  // to the pool behind an event recorded there when the last view is dropped
+  hipStream_t ustream = nullptr;
+  std::mutex fence_mu;
+  std::shared_ptr<Fence> fence;
+  void fence_on_consumer();
   ~Result();
 };
 
@@ -167,7 +187,7 @@ class Engine {
   Tuning tuning_for(const Request& req);
 
   void* dev_alloc(size_t bytes);
-  void dev_release(void* p, size_t bytes);
+  void dev_release(void* p, size_t bytes, std::shared_ptr<Fence> fence = nullptr);
   StageTimes last_times() const { return times_; }
   void set_timing(bool on) { timing_ = on; }
   std::mutex& mutex() { return mu_; }
@@ -185,7 +205,11 @@ class Engine {
   hipStream_t stream_ = nullptr;
   std::mutex mu_;
   std::mutex pool_mu_;
-  std::multimap<size_t, void*> pool_;
+  struct PoolBlock {
+    void* p;
+    std::shared_ptr<Fence> fence;  // null: free for use at once
+  };
+  std::multimap<size_t, PoolBlock> pool_;
   struct Scratch {
     void* p = nullptr;
     size_t cap = 0;

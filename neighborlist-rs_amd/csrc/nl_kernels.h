@@ -33,6 +33,7 @@ struct PairArgs {
   int64_t capacity;          // edges the output buffers hold (fill pass)
   int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index;
   int32_t per_sys_rc;        // rc2 is [n_sys][k] (per-system cutoffs)
+  int32_t full;              // full lists: every pair in both directions
   int32_t group;             // home atoms per cell-list work item (<= 64)
   int32_t parts;             // slices of the half stencil per home group
 };

@@ -613,7 +613,14 @@ using Acc = typename std::conditional<FILL, int64_t, uint32_t>::type;
 
 // Emit or count one step of up to 64 candidates; `mem` holds the lane's
 // list bits (zero for lanes without a pair).
-template <bool FILL, int NQ>
+//
+// FULL (fill pass of full lists): each pair is written twice, (i, j, S) at
+// an even slot e and (j, i, -S) at e + 1.  Every count is then even, so every
+// list bound and item offset is even and the pair goes out as 16-byte
+// (edge, distance) and 8-byte (shift) stores.  A self image (i, i, S) is
+// emitted once by the half stencil and mirrored to (i, i, -S), which the
+// half list writes as a second emit instead.
+template <bool FILL, int NQ, bool FULL = false>
 __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
                                  int64_t gj, int sx, int sy, int sz, double d2, Acc<FILL>* cur) {
   if (!FILL) {
@@ -630,21 +637,33 @@ __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, 
     if (FILL && on) {
       // plain stores: L2 merges the partial lines of successive steps
       // (non-temporal stores measured 10% slower here)
-      const int64_t e = cur[g] + lanes_below(m);
-      int64_t* pi = A.edges + (C.base[g] + e);
-      pi[0] = gi;
-      pi[C.len[g]] = gj;
-      int32_t* q = A.shifts + 3 * e;  // one address, three offsets
-      q[0] = sx;
-      q[1] = sy;
-      q[2] = sz;
-      if (A.dist) A.dist[e] = d2;  // squared here; square roots in a pass of their own
+      if (FULL) {
+        const int64_t e = cur[g] + 2 * (int64_t)lanes_below(m);
+        int64_t* pi = A.edges + (C.base[g] + e);
+        *reinterpret_cast<longlong2*>(pi) = make_longlong2(gi, gj);
+        *reinterpret_cast<longlong2*>(pi + C.len[g]) = make_longlong2(gj, gi);
+        int2* q = reinterpret_cast<int2*>(A.shifts + 3 * e);
+        q[0] = make_int2(sx, sy);
+        q[1] = make_int2(sz, -sx);
+        q[2] = make_int2(-sy, -sz);
+        if (A.dist) *reinterpret_cast<double2*>(A.dist + e) = make_double2(d2, d2);
+      } else {
+        const int64_t e = cur[g] + lanes_below(m);
+        int64_t* pi = A.edges + (C.base[g] + e);
+        pi[0] = gi;
+        pi[C.len[g]] = gj;
+        int32_t* q = A.shifts + 3 * e;  // one address, three offsets
+        q[0] = sx;
+        q[1] = sy;
+        q[2] = sz;
+        if (A.dist) A.dist[e] = d2;  // squared here; square roots in a pass of their own
+      }
     }
-    cur[g] += (int64_t)__popcll(m);
+    cur[g] += (FULL ? 2 : 1) * (int64_t)__popcll(m);
   }
 }
 
-template <bool FILL, int NQ>
+template <bool FILL, int NQ, bool FULL>
 __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
                                 int mode, Acc<FILL>* cur) {
   const int lane = lane_id();
@@ -695,7 +714,7 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
     const uint32_t mem = valid ? members64(d2, C) : 0u;
     if (FILL && __ballot(mem != 0) == 0) continue;
     const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
-    emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
+    emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
   }
 }
 
@@ -724,7 +743,7 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 // orientation (lower original index first, S = s_j - s_i + bs from it,
 // search.rs:695-697); a self image (same atom, bs != 0) is written for both
 // S and -S, as the reference emits both (search.rs:549).
-template <bool FILL, int NQ, bool F32>
+template <bool FILL, int NQ, bool F32, bool FULL>
 __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
                           int part, WaveLds& L, Acc<FILL>* cur) {
   const int lane = lane_id();
@@ -920,9 +939,11 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             const bool cand = i < lim;
             if (!FILL && !C.disjoint) {
               // Count pass, cumulative lists: lane counters, no ballots.  A
-              // self image (same atom, other image) counts twice (both S and
-              // -S are emitted).  Lists past q_cnt have -inf thresholds.
-              const uint32_t w1 = cand ? 1u + (uint32_t)(oj == oi) : 0u;
+              // self image (same atom, other image) counts twice in a half
+              // list (both S and -S are emitted); full lists count every hit
+              // once and double the item's count at the end.  Lists past
+              // q_cnt have -inf thresholds.
+              const uint32_t w1 = cand ? 1u + (uint32_t)(!FULL && oj == oi) : 0u;
               if (F32) {
                 const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
                 const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -1003,8 +1024,11 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                 gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
               }
             }
-            emit_step<FILL, NQ>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
-            if (!FILL || __ballot(self_img))
+            emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
+            // the self image's -S copy (full lists get it from the mirror;
+            // their count pass counts each half-stencil hit once, doubled
+            // when the item ends)
+            if (!FULL && (!FILL || __ballot(self_img)))
               emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, d2, cur);
           }
         }
@@ -1022,7 +1046,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
 // takes the cell-list path; slots that do not start a group are empty).
 // Edges of one system are thus contiguous and systems come out in input
 // order, as batch.rs:135-144 concatenates them.
-template <bool FILL, int NQ, bool F32>
+template <bool FILL, int NQ, bool F32, bool FULL>
 __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds& L) {
   const int s = uniform_i32(find_sys(A.item_base, A.n_sys, u));
   const SysDesc* d = A.sys + s;
@@ -1053,11 +1077,11 @@ __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds&
   }
   const int n_atoms = d->n_atoms;
   if (local < n_atoms) {
-    if (mode != kCellList) unit_bruteforce<FILL, NQ>(A, C, d, d->atom_start + local, mode, cur);
+    if (mode != kCellList) unit_bruteforce<FILL, NQ, FULL>(A, C, d, d->atom_start + local, mode, cur);
   } else if (mode == kCellList) {
     const int64_t r = local - n_atoms;
     const int64_t p = d->atom_start + r / A.parts;
-    unit_cell<FILL, NQ, F32>(A, C, d, p, (int)(r % A.parts), L, cur);
+    unit_cell<FILL, NQ, F32, FULL>(A, C, d, p, (int)(r % A.parts), L, cur);
   }
   if (!FILL) {
 #pragma unroll
@@ -1065,6 +1089,7 @@ __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds&
       uint32_t t = cur[g];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      if (FULL) t *= 2u;  // both directions of every half-stencil hit
       if (g < A.q_cnt && lane_id() == 0) A.counts[(int64_t)(A.q_lo + g) * A.n + u] = t;
     }
   }
@@ -1128,7 +1153,8 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 // One wave per listed work item (a loop over several items per wave keeps
 // the per-item state live across iterations and costs about 40 VGPRs).
 // F32: FP32 filter with FP64 recheck in the band (cell-list items).
-template <bool FILL, int NQ, bool F32>
+// FULL: full lists (count and fill passes).
+template <bool FILL, int NQ, bool F32, bool FULL>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   __shared__ WaveLds s_lds[kWaves];
   const int w = threadIdx.x >> 6;
@@ -1139,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   // fill pass launched before the host saw the counts: write nothing when
   // the lists do not fit; the host re-runs the fill with exact buffers
   if (FILL && A.bounds[A.k] > A.capacity) return;
-  pair_item<FILL, NQ, F32>(A, u, s_lds[w]);
+  pair_item<FILL, NQ, F32, FULL>(A, u, s_lds[w]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1308,17 +1334,25 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
 template <int NQ>
 static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
   const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
+#define NL_PAIRS(FILL, F32)                                                \
+  do {                                                                     \
+    if (a.full)                                                            \
+      hipLaunchKernelGGL((k_pairs<FILL, NQ, F32, true>), g, b, 0, st, a);  \
+    else                                                                   \
+      hipLaunchKernelGGL((k_pairs<FILL, NQ, F32, false>), g, b, 0, st, a); \
+  } while (0)
   if (a.use_fp32) {
     if (fill)
-      hipLaunchKernelGGL((k_pairs<true, NQ, true>), g, b, 0, st, a);
+      NL_PAIRS(true, true);
     else
-      hipLaunchKernelGGL((k_pairs<false, NQ, true>), g, b, 0, st, a);
+      NL_PAIRS(false, true);
   } else {
     if (fill)
-      hipLaunchKernelGGL((k_pairs<true, NQ, false>), g, b, 0, st, a);
+      NL_PAIRS(true, false);
     else
-      hipLaunchKernelGGL((k_pairs<false, NQ, false>), g, b, 0, st, a);
+      NL_PAIRS(false, false);
   }
+#undef NL_PAIRS
 }
 
 hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const SysDesc* sys,

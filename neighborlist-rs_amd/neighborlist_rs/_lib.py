@@ -54,6 +54,9 @@ _SIGS = {
     "nl_search_systems": (C.c_int, [dp, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp,
                                     C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     res_pp]),
+    "nl_search_ex": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp,
+                               C.c_size_t, C.c_int, C.c_int, C.c_uint, C.c_void_p, res_pp]),
+    "nl_result_dlpack": (C.c_int, [res_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     "nl_result_system_ranges": (C.c_int, [res_p, i64p, i32p]),
     "nl_result_copy_dist": (C.c_int, [res_p, C.c_size_t, dp]),
     "nl_set_timing": (None, [C.c_int]),

@@ -4,6 +4,14 @@ import sys
 import numpy as np
 import pytest
 
+try:
+    # loaded before the engine library, so that the engine binds to torch's
+    # bundled HIP runtime and the process holds one runtime (torch's device
+    # tensors and the engine's lists then live side by side)
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "neighborlist-rs_amd")
 for p in (ROOT, PKG, os.path.dirname(os.path.abspath(__file__))):

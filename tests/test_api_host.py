@@ -181,3 +181,29 @@ def test_thresholds_and_threads(nr):
     nr.init_logging("debug")
     nr.init_logging("info")
     nr.init_logging(None)
+
+
+# ---- torch surface (neighborlist_rs.tensors): validation before device work --
+def test_torch_surface_validation(nr):
+    import torch
+    from neighborlist_rs import tensors
+    with pytest.raises(TypeError, match="float64"):
+        nr.build_neighborlists_torch(None, torch.zeros((4, 3), dtype=torch.float32), 2.0)
+    with pytest.raises(ValueError, match=r"Positions must be \(N, 3\)"):
+        nr.build_neighborlists_torch(None, torch.zeros((4, 2), dtype=torch.float64), 2.0)
+    with pytest.raises(ValueError, match="device"):
+        nr.build_neighborlists_torch(None, torch.zeros((4, 3), dtype=torch.float64), 2.0)
+    with pytest.raises(TypeError, match="torch.Tensor"):
+        nr.build_neighborlists_torch(None, np.zeros((4, 3)), 2.0)
+    # batch ids: maximal runs of equal ids, non-decreasing, same length
+    np.testing.assert_array_equal(tensors._batch_sizes(torch.tensor([0, 0, 1, 1, 1, 4]), 6),
+                                  [2, 3, 1])
+    np.testing.assert_array_equal(tensors._batch_sizes(np.array([7, 7, 7]), 3), [3])
+    with pytest.raises(ValueError, match="monotonic"):
+        tensors._batch_sizes(torch.tensor([0, 1, 0]), 3)
+    with pytest.raises(ValueError, match="same length"):
+        tensors._batch_sizes(torch.tensor([0, 1]), 3)
+    with pytest.raises(ValueError, match="Length of labels"):
+        tensors._check_cutoffs([1.0, 2.0], ["a"])
+    with pytest.raises(ValueError, match="All cutoffs must be positive and finite"):
+        tensors._check_cutoffs([1.0, float("nan")], None)
