@@ -86,6 +86,20 @@ def run_systems(lib, L, dpos, n, sizes, H9, has, pbc, cut, multi):
     return edges
 
 
+def run_ex(lib, L, dpos, n, sizes, H9, has, pbc, cut, multi, opts):
+    """nl_search_ex on device positions (options: NL_* bits of nl_engine_ext.h)."""
+    r = C.c_void_p()
+    rc = lib.nl_search_ex(
+        C.c_void_p(dpos), n, sizes.ctypes.data_as(L.i64p), sizes.size,
+        H9.ctypes.data_as(L.dp), has.ctypes.data_as(L.u8p), pbc.ctypes.data_as(L.u8p),
+        cut.ctypes.data_as(L.dp), cut.size, 0, int(multi), opts, None, C.byref(r))
+    if rc != 0:
+        raise RuntimeError(L.last_error())
+    edges = sum(int(lib.nl_result_num_edges(r, q)) for q in range(cut.size))
+    lib.nl_result_free(r)
+    return edges
+
+
 def timing(lib):
     b, c, f, t = C.c_float(), C.c_float(), C.c_float(), C.c_float()
     e, nb = C.c_int64(), C.c_int64()
@@ -397,6 +411,30 @@ def main():
         dpos.free()
         return
 
+    # full (directed) lists of the same batch: the pair kernels write both
+    # directions of every pair (56 B per half-list pair), no second pass
+    NL_DEVICE_POSITIONS, NL_FULL = 1, 16
+    for _ in range(2):
+        run_ex(lib, L, dpos.ptr, n, sizes, H9, has, pbc, cut, True, NL_DEVICE_POSITIONS | NL_FULL)
+    ffill, fcount, ftot = [], [], []
+    for _ in range(max(5, args.steps // 2)):
+        full_edges = run_ex(lib, L, dpos.ptr, n, sizes, H9, has, pbc, cut, True,
+                            NL_DEVICE_POSITIONS | NL_FULL)
+        b, c, f, t, _ = timing(lib)
+        fcount.append(c)
+        ffill.append(f)
+        ftot.append(t)
+    full_fill = float(np.mean(ffill))
+    full_lists = {"edges_per_step": int(full_edges),
+                  "build_and_count_ms": round(float(np.mean(fcount)), 4),
+                  "fill_ms": round(full_fill, 4),
+                  "device_ms": round(float(np.mean(ftot)), 4),
+                  "edges_per_s": round(full_edges / (float(np.mean(ftot)) * 1e-3), 1),
+                  "fill_alg_GBps": round((28.0 * full_edges + 24.0 * n) / (full_fill * 1e-3) / 1e9,
+                                         2),
+                  "timing": "hipEvents around the device stages of the eager path"}
+    log(f"full lists: {full_edges} edges, fill {full_fill:.3f} ms")
+
     # ---- plain path: one call per system (same inputs, device-resident) --
     starts = np.concatenate([[0], np.cumsum(sizes)])
     one = np.ones(1, np.int64)
@@ -502,6 +540,7 @@ def main():
                              "speedup_vs_cpu": None if c3_cpu_ms is None
                              else round(c3_cpu_ms / c3_ms, 2)},
         "precision_c3_rc6": precision,
+        "full_lists_c4": full_lists,
         "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -40,6 +40,11 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
 }
 
 __device__ inline int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uniform_i64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 __device__ inline int find_sys(const int64_t* sys_start, int n_sys, int64_t a) {
   int lo = 0, hi = n_sys - 1;
@@ -515,7 +520,7 @@ __global__ __launch_bounds__(256) void k_finalize(
   matvec3(d->h, (double)(-si[0]), (double)(-si[1]), (double)(-si[2]), sv);
   const double wx = q[0] - sv[0], wy = q[1] - sv[1], wz = q[2] - sv[2];
   const int32_t loc = (int32_t)(a - d->atom_start);
-  spos64[p] = Pos64{wx, wy, wz, (double)loc};
+  spos64[p] = Pos64{wx, wy, wz, __hiloint2double(0, loc)};  // w: index bits
   spos32[p] = make_float4((float)wx, (float)wy, (float)wz, __int_as_float(loc));
   sidx[p] = make_int4(si[0], si[1], si[2], loc);
 }
@@ -573,20 +578,24 @@ struct Cuts {
   __device__ bool has(int g) const { return NQ < 8 || g < cnt; }
 };
 
-template <int NQ>
+template <bool DISJ, int NQ>
 __device__ __forceinline__ uint32_t members64(double d2, const Cuts<NQ>& C) {
   uint32_t m = 0;
 #pragma unroll
   for (int g = 0; g < NQ; ++g)
     if (C.has(g) && d2 < C.r[g]) m |= 1u << g;
-  if (C.disjoint) m = (d2 < C.prev) ? 0u : (m & (0u - m));
+  if (DISJ) m = (d2 < C.prev) ? 0u : (m & (0u - m));
   return m;
+}
+template <int NQ>
+__device__ __forceinline__ uint32_t members64_rt(double d2, const Cuts<NQ>& C) {
+  return C.disjoint ? members64<true, NQ>(d2, C) : members64<false, NQ>(d2, C);
 }
 
 // FP32 filter: the decision for each threshold is certain when e2 lies
 // below lo (inside) or above hi (outside); `unsure` when any needed
 // threshold is within its band, then the FP64 expression decides.
-template <int NQ>
+template <bool DISJ, int NQ>
 __device__ __forceinline__ uint32_t members32(float e2, const Cuts<NQ>& C, bool* unsure) {
   uint32_t m = 0;
   bool u = false;
@@ -597,7 +606,7 @@ __device__ __forceinline__ uint32_t members32(float e2, const Cuts<NQ>& C, bool*
       u |= (e2 >= C.lo[g]) && (e2 <= C.hi[g]);
     }
   }
-  if (C.disjoint) {
+  if (DISJ) {
     u |= (e2 >= C.prev_lo) && (e2 <= C.prev_hi);
     m = (e2 < C.prev_lo) ? 0u : (m & (0u - m));
   }
@@ -637,26 +646,27 @@ __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, 
     if (FILL && on) {
       // plain stores: L2 merges the partial lines of successive steps
       // (non-temporal stores measured 10% slower here)
+      // wave-uniform bases (the cursor is uniform) plus a 32-bit lane offset
+      const uint32_t lb = lanes_below(m);
+      int64_t* src = A.edges + (C.base[g] + cur[g]);
+      int64_t* dst = src + C.len[g];
+      int32_t* sh = A.shifts + 3 * cur[g];
       if (FULL) {
-        const int64_t e = cur[g] + 2 * (int64_t)lanes_below(m);
-        int64_t* pi = A.edges + (C.base[g] + e);
-        *reinterpret_cast<longlong2*>(pi) = make_longlong2(gi, gj);
-        *reinterpret_cast<longlong2*>(pi + C.len[g]) = make_longlong2(gj, gi);
-        int2* q = reinterpret_cast<int2*>(A.shifts + 3 * e);
+        reinterpret_cast<longlong2*>(src)[lb] = make_longlong2(gi, gj);
+        reinterpret_cast<longlong2*>(dst)[lb] = make_longlong2(gj, gi);
+        int2* q = reinterpret_cast<int2*>(sh) + 3u * lb;
         q[0] = make_int2(sx, sy);
         q[1] = make_int2(sz, -sx);
         q[2] = make_int2(-sy, -sz);
-        if (A.dist) *reinterpret_cast<double2*>(A.dist + e) = make_double2(d2, d2);
+        if (A.dist) reinterpret_cast<double2*>(A.dist + cur[g])[lb] = make_double2(d2, d2);
       } else {
-        const int64_t e = cur[g] + lanes_below(m);
-        int64_t* pi = A.edges + (C.base[g] + e);
-        pi[0] = gi;
-        pi[C.len[g]] = gj;
-        int32_t* q = A.shifts + 3 * e;  // one address, three offsets
+        src[lb] = gi;
+        dst[lb] = gj;
+        int32_t* q = sh + 3u * lb;  // one address, three offsets
         q[0] = sx;
         q[1] = sy;
         q[2] = sz;
-        if (A.dist) A.dist[e] = d2;  // squared here; square roots in a pass of their own
+        if (A.dist) A.dist[cur[g] + lb] = d2;  // squared here; square roots in a pass of their own
       }
     }
     cur[g] += (FULL ? 2 : 1) * (int64_t)__popcll(m);
@@ -711,7 +721,7 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
       }
       d2 = norm2(dx, dy, dz);
     }
-    const uint32_t mem = valid ? members64(d2, C) : 0u;
+    const uint32_t mem = valid ? members64_rt(d2, C) : 0u;
     if (FILL && __ballot(mem != 0) == 0) continue;
     const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
     emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
@@ -863,7 +873,9 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int total = __shfl(incl, 63, 64);
+      // readlane, not a shuffle: a scalar, so the chunk loop below is uniform
+      // (the fill cursors it carries stay scalar)
+      const int total = __builtin_amdgcn_readlane(incl, 63);
       if (total > 0) {
         // Candidates stream through the lanes 64 at a time.  The loads of
         // the next chunk are issued before the current one is tested, so
@@ -922,115 +934,124 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             py = c32.y + L.off32[1][csg];
             pz = c32.z + L.off32[2][csg];
           }
-          const int oj = F32 ? __float_as_int(c32.w) : (int)c.w;
+          const int oj = F32 ? __float_as_int(c32.w) : __double2loint(c.w);
           // home atom i is tested against this lane's candidate iff i < lim
           // (in the home bin without image only the later slots)
           const int lim = cvalid ? (home_seg ? cjs - hb : 64) : 0;
+          // the home-atom loop, compiled once per disjoint mode (a uniform
+          // branch inside it costs register moves and selects on every step)
+          auto home_loop = [&](auto disj_tag) {
+            constexpr bool DISJ = decltype(disj_tag)::value;
 #pragma unroll 1
-          for (int i = 0; i < hn; ++i) {
-            // broadcast LDS reads of the home atom (uniform address)
-            float4 h32 = make_float4(0.f, 0.f, 0.f, 0.f);
-            Pos64 h64 = {0.0, 0.0, 0.0, 0.0};
-            if (F32)
-              h32 = L.home32[i];
-            else
-              h64 = L.home64[i];
-            const int oi = F32 ? __float_as_int(h32.w) : (int)h64.w;
-            const bool cand = i < lim;
-            if (!FILL && !C.disjoint) {
-              // Count pass, cumulative lists: lane counters, no ballots.  A
-              // self image (same atom, other image) counts twice in a half
-              // list (both S and -S are emitted); full lists count every hit
-              // once and double the item's count at the end.  Lists past
-              // q_cnt have -inf thresholds.
-              const uint32_t w1 = cand ? 1u + (uint32_t)(!FULL && oj == oi) : 0u;
+            for (int i = 0; i < hn; ++i) {
+              // broadcast LDS reads of the home atom (uniform address)
+              float4 h32 = make_float4(0.f, 0.f, 0.f, 0.f);
+              Pos64 h64 = {0.0, 0.0, 0.0, 0.0};
+              if (F32)
+                h32 = L.home32[i];
+              else
+                h64 = L.home64[i];
+              const int oi = F32 ? __float_as_int(h32.w) : __double2loint(h64.w);
+              const bool cand = i < lim;
+              if (!FILL && !DISJ) {
+                // Count pass, cumulative lists: lane counters, no ballots.  A
+                // self image (same atom, other image) counts twice in a half
+                // list (both S and -S are emitted); full lists count every hit
+                // once and double the item's count at the end.  Lists past
+                // q_cnt have -inf thresholds.
+                const uint32_t w1 = cand ? 1u + (uint32_t)(!FULL && oj == oi) : 0u;
+                if (F32) {
+                  const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
+                  const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                  bool unsure = false;
+#pragma unroll
+                  for (int g = 0; g < NQ; ++g) {
+                    const bool acc = e2 < C.lo[g];
+                    cur[g] += acc ? w1 : 0u;
+                    unsure |= !acc && e2 <= C.hi[g];
+                  }
+                  unsure = unsure && cand;
+                  if (__ballot(unsure)) {
+                    const Pos64 h64 = A.spos64[hb + i];  // uniform address
+                    const double xi64 = h64.x, yi64 = h64.y, zi64 = h64.z;
+                    if (unsure) {
+                      // FP64 recheck: the reference expression, exactly
+                      const Pos64 c64 = A.spos64[cjs];
+                      const double d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy,
+                                              (c64.z - zi64) + coz);
+#pragma unroll
+                      for (int g = 0; g < NQ; ++g)
+                        cur[g] += (!(e2 < C.lo[g]) && d2 < C.r[g]) ? w1 : 0u;
+                    }
+                  }
+                } else {
+                  // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
+                  const double d2 =
+                      norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
+#pragma unroll
+                  for (int g = 0; g < NQ; ++g) cur[g] += d2 < C.r[g] ? w1 : 0u;
+                }
+                continue;
+              }
+              uint32_t mem = 0;
+              double d2 = INFINITY;
               if (F32) {
                 const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
                 const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 bool unsure = false;
-#pragma unroll
-                for (int g = 0; g < NQ; ++g) {
-                  const bool acc = e2 < C.lo[g];
-                  cur[g] += acc ? w1 : 0u;
-                  unsure |= !acc && e2 <= C.hi[g];
-                }
-                unsure = unsure && cand;
-                if (__ballot(unsure)) {
+                mem = members32<DISJ, NQ>(e2, C, &unsure);
+                mem = cand ? mem : 0u;
+                const bool need64 = cand && (unsure || (A.dist != nullptr && mem != 0));
+                if (__ballot(need64)) {
                   const Pos64 h64 = A.spos64[hb + i];  // uniform address
                   const double xi64 = h64.x, yi64 = h64.y, zi64 = h64.z;
-                  if (unsure) {
+                  if (need64) {
                     // FP64 recheck: the reference expression, exactly
                     const Pos64 c64 = A.spos64[cjs];
-                    const double d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy,
-                                            (c64.z - zi64) + coz);
-#pragma unroll
-                    for (int g = 0; g < NQ; ++g)
-                      cur[g] += (!(e2 < C.lo[g]) && d2 < C.r[g]) ? w1 : 0u;
+                    d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy, (c64.z - zi64) + coz);
+                    mem = members64<DISJ, NQ>(d2, C);
                   }
                 }
               } else {
                 // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-                const double d2 =
-                    norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
-#pragma unroll
-                for (int g = 0; g < NQ; ++g) cur[g] += d2 < C.r[g] ? w1 : 0u;
+                d2 = norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
+                if (cand) mem = members64<DISJ, NQ>(d2, C);
               }
-              continue;
-            }
-            uint32_t mem = 0;
-            double d2 = INFINITY;
-            if (F32) {
-              const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
-              const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              bool unsure = false;
-              mem = members32(e2, C, &unsure);
-              mem = cand ? mem : 0u;
-              const bool need64 = cand && (unsure || (A.dist != nullptr && mem != 0));
-              if (__ballot(need64)) {
-                const Pos64 h64 = A.spos64[hb + i];  // uniform address
-                const double xi64 = h64.x, yi64 = h64.y, zi64 = h64.z;
-                if (need64) {
-                  // FP64 recheck: the reference expression, exactly
-                  const Pos64 c64 = A.spos64[cjs];
-                  d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy, (c64.z - zi64) + coz);
-                  mem = members64(d2, C);
+              if (FILL && __ballot(mem != 0) == 0) continue;
+              const bool self_img = (mem != 0) && (oj == oi);
+              int sx = 0, sy = 0, sz = 0;
+              int64_t gi = 0, gj = 0;
+              if (FILL) {
+                const int4 hsi = L.homesi[i];
+                const int six = hsi.x, siy = hsi.y, siz = hsi.z;
+                if (mem) {
+                  sx = cj.x - six + sbx;
+                  sy = cj.y - siy + sby;
+                  sz = cj.z - siz + sbz;
+                  int lo_i = oi, hi_i = oj;
+                  if (oj < oi) {
+                    lo_i = oj;
+                    hi_i = oi;
+                    sx = -sx;
+                    sy = -sy;
+                    sz = -sz;
+                  }
+                  gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
+                  gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
                 }
               }
-            } else {
-              // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-              d2 = norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
-              if (cand) mem = members64(d2, C);
+              emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
+              // the self image's -S copy (full lists get it from the mirror;
+              // their count pass counts each half-stencil hit once, doubled
+              // when the item ends)
+              if (!FULL && (!FILL || __ballot(self_img)))
+                emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, d2, cur);
             }
-            if (FILL && __ballot(mem != 0) == 0) continue;
-            const bool self_img = (mem != 0) && (oj == oi);
-            int sx = 0, sy = 0, sz = 0;
-            int64_t gi = 0, gj = 0;
-            if (FILL) {
-              const int4 hsi = L.homesi[i];
-              const int six = hsi.x, siy = hsi.y, siz = hsi.z;
-              if (mem) {
-                sx = cj.x - six + sbx;
-                sy = cj.y - siy + sby;
-                sz = cj.z - siz + sbz;
-                int lo_i = oi, hi_i = oj;
-                if (oj < oi) {
-                  lo_i = oj;
-                  hi_i = oi;
-                  sx = -sx;
-                  sy = -sy;
-                  sz = -sz;
-                }
-                gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
-                gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
-              }
-            }
-            emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
-            // the self image's -S copy (full lists get it from the mirror;
-            // their count pass counts each half-stencil hit once, doubled
-            // when the item ends)
-            if (!FULL && (!FILL || __ballot(self_img)))
-              emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, d2, cur);
-          }
+          };
+          if (C.disjoint)
+            home_loop(std::true_type{});
+          else
+            home_loop(std::false_type{});
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -1157,11 +1178,12 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 template <bool FILL, int NQ, bool F32, bool FULL>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   __shared__ WaveLds s_lds[kWaves];
-  const int w = threadIdx.x >> 6;
+  const int w = uniform_i32(threadIdx.x >> 6);
   const int64_t k = (int64_t)blockIdx.x * kWaves + w;
   const int64_t front = A.n_real[0];
   if (k >= front + (int64_t)A.n_real[1]) return;
-  const int64_t u = k < front ? A.list[k] : A.list[A.n - 1 - (k - front)];
+  // wave-uniform item: its loads are scalar and the fill cursors stay scalar
+  const int64_t u = uniform_i64(k < front ? A.list[k] : A.list[A.n - 1 - (k - front)]);
   // fill pass launched before the host saw the counts: write nothing when
   // the lists do not fit; the host re-runs the fill with exact buffers
   if (FILL && A.bounds[A.k] > A.capacity) return;
