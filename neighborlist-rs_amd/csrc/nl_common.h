@@ -18,6 +18,9 @@ constexpr int kWaves = 4;           // waves (units) per pair-kernel block
 constexpr int kRadixMaxBits = 12;   // LSD radix digit width limit (4096 digits)
 constexpr int kRadixMaxDigits = 1 << kRadixMaxBits;
 constexpr int kRadixTile = 2048;    // keys per radix / scan tile (256 threads x 8)
+// scans of up to this many tiles use one decoupled look-back launch; longer
+// ones reduce-then-scan (three launches, no block waits on another)
+constexpr int kScanChainTiles = 32;
 
 // Search mode chosen per system, exactly as single.rs:60-89 / batch.rs:100-124.
 enum Mode : int32_t {

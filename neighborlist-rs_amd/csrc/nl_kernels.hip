@@ -1264,6 +1264,109 @@ __global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
   }
 }
 
+// The same scan for long inputs, reduce-then-scan: tile sums, one
+// exclusive scan over them, then each tile's own scan from its offset.  No
+// block waits on another (a look-back chain across hundreds of tiles costs
+// more than the two extra launches).
+__device__ inline uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ in, int64_t len,
+                                                   uint64_t* __restrict__ sums) {
+  __shared__ uint64_t s_w[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t e0 = (int64_t)blockIdx.x * kRadixTile + 8 * tid;
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sum += (e0 + i < len) ? (uint64_t)in[e0 + i] : 0ull;
+  sum = wave_sum_u64(sum);
+  if (lane == 0) s_w[w] = sum;
+  __syncthreads();
+  if (tid == 0) sums[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// In-place exclusive scan of the n tile sums, one block, 4096 per round.
+__global__ __launch_bounds__(1024) void k_scan_sums_excl(uint64_t* __restrict__ sums, int64_t n) {
+  __shared__ uint64_t s_w[16];
+  __shared__ uint64_t s_carry;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t b = 0; b < n; b += 4096) {
+    const int64_t e0 = b + 4 * tid;
+    uint64_t v[4];
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = (e0 + i < n) ? sums[e0 + i] : 0ull;
+      t += v[i];
+    }
+    const uint64_t winc = wave_incl_scan(t);
+    if (lane == 63) s_w[w] = winc;
+    __syncthreads();
+    uint64_t run = s_carry + winc - t;
+    uint64_t all = 0;
+    for (int i = 0; i < 16; ++i) {
+      if (i < w) run += s_w[i];
+      all += s_w[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (e0 + i < n) sums[e0 + i] = run;
+      run += v[i];
+    }
+    __syncthreads();
+    if (tid == 0) s_carry += all;
+    __syncthreads();
+  }
+}
+
+// Tile `blockIdx.x` of the scan, starting from its offset pre[tile].
+// bounds[q] = out[q * seg] for q < nseg; bounds[nseg] = out[len] = total.
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in,
+                                                    int64_t* __restrict__ out, int64_t len,
+                                                    int64_t seg, int nseg,
+                                                    int64_t* __restrict__ bounds,
+                                                    const uint64_t* __restrict__ pre) {
+  __shared__ uint64_t s_w[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
+  const int64_t e0 = t0 + 8 * tid;
+  uint64_t v[8];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    v[i] = (e0 + i < len) ? (uint64_t)in[e0 + i] : 0ull;
+    sum += v[i];
+  }
+  const uint64_t winc = wave_incl_scan(sum);
+  if (lane == 63) s_w[w] = winc;
+  __syncthreads();
+  uint64_t run = pre[blockIdx.x] + winc - sum;
+  for (int i = 0; i < w; ++i) run += s_w[i];
+  // list bounds that fall in this tile (block-uniform range of q)
+  const int64_t t1 = min(t0 + (int64_t)kRadixTile, len);
+  const int64_t q_lo = (t0 + seg - 1) / seg;
+  const int64_t q_hi = min((int64_t)nseg, (t1 + seg - 1) / seg);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int64_t e = e0 + i;
+    if (e < len) {
+      out[e] = (int64_t)run;
+      for (int64_t q = q_lo; q < q_hi; ++q)
+        if (q * seg == e) bounds[q] = (int64_t)run;
+      run += v[i];
+      if (e == len - 1) {
+        out[len] = (int64_t)run;
+        bounds[nseg] = (int64_t)run;
+      }
+    }
+  }
+}
+
 // Per-system edge ranges: ranges[(s*k + q)*2 + {0,1}].
 __global__ void k_sys_ranges(const int64_t* __restrict__ off,
                              const int64_t* __restrict__ item_base, int n_sys, int k, int64_t n,
@@ -1401,8 +1504,17 @@ hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st) {
 hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
                        int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st) {
   if (len <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan, dim3((unsigned)radix_tiles(len)), dim3(256), 0, st, in, out, len,
-                     seg, nseg, bounds, status, ticket);
+  const int64_t tiles = radix_tiles(len);
+  if (tiles <= kScanChainTiles) {
+    hipLaunchKernelGGL(k_scan, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len, seg, nseg,
+                       bounds, status, ticket);
+    return hipGetLastError();
+  }
+  // status (one u64 per tile) holds the tile sums, then their prefixes
+  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)tiles), dim3(256), 0, st, in, len, status);
+  hipLaunchKernelGGL(k_scan_sums_excl, dim3(1), dim3(1024), 0, st, status, tiles);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len, seg,
+                     nseg, bounds, (const uint64_t*)status);
   return hipGetLastError();
 }
 
