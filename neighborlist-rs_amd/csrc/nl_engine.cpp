@@ -613,13 +613,12 @@ Tuning Engine::tuning_for(const Request& req) {
   probe.want_ranges = false;
   probe.want_dist = false;
   // device time of the search stages (hipEvents around the launches), best
-  // of three runs; a candidate must beat the incumbent by 2% to replace it,
-  // so timing noise does not walk the choice away from the default
-  auto time_one = [&](const Tuning& x) -> double {
+  // of `reps` runs
+  auto time_one = [&](const Tuning& x, int reps) -> double {
     double best = 1e30;
     const bool was = timing_;
     timing_ = true;
-    for (int rep = 0; rep < 3; ++rep) {
+    for (int rep = 0; rep < reps; ++rep) {
       Result* r = nullptr;
       if (run_with(probe, x, &r) != 0) {
         timing_ = was;
@@ -631,40 +630,48 @@ Tuning Engine::tuning_for(const Request& req) {
     timing_ = was;
     return best;
   };
+  // bin size first (a candidate must beat the incumbent by 2%, so timing
+  // noise does not walk the choice away from the default) ...
   Tuning best = t;
-  double tb = time_one(best);
-  auto consider = [&](Tuning x) {
-    const double tx = time_one(x);
-    if (tx < 0.98 * tb) {
-      tb = tx;
-      best = x;
-    }
-  };
+  double tb = time_one(best, 3);
   for (int dv : {1, 2, 3, 4}) {
     if (dv == t.cell_div) continue;
     Tuning x = best;
     x.cell_div = dv;
-    consider(x);
+    const double tx = time_one(x, 3);
+    if (tx < 0.98 * tb) {
+      tb = tx;
+      best = x;
+    }
   }
-  const Tuning after_div = best;
-  for (int g : {1, 2, 8, 16}) {
-    if (g == after_div.group) continue;
-    Tuning x = after_div;
-    x.group = g;
-    consider(x);
-  }
-  const Tuning after_g = best;
-  for (int p : {1, 2, 4}) {
-    if (p == after_g.parts) continue;
-    Tuning x = after_g;
-    x.parts = p;
-    consider(x);
-  }
+  // ... then home-group size and stencil slices jointly (they interact: more
+  // slices suit larger groups), FP32 filter on and off for the best pair ...
+  std::vector<std::pair<double, Tuning>> cand;
+  for (int g : {2, 4, 8, 16})
+    for (int p : {1, 2, 4}) {
+      Tuning x = best;
+      x.group = g;
+      x.parts = p;
+      cand.emplace_back(time_one(x, 3), x);
+    }
+  std::sort(cand.begin(), cand.end(),
+            [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) {
+              return a.first < b.first;
+            });
   {
-    Tuning x = best;
-    x.use_fp32 = 1 - best.use_fp32;
-    consider(x);
+    Tuning x = cand[0].second;
+    x.use_fp32 = 1 - x.use_fp32;
+    cand.insert(cand.begin() + 1, {time_one(x, 3), x});
   }
+  // ... and the leading three re-timed in turn, best of five each, twice
+  const int top = std::min<int>(3, (int)cand.size());
+  std::vector<double> tt(top, 1e30);
+  for (int round = 0; round < 2; ++round)
+    for (int i = 0; i < top; ++i) tt[i] = std::min(tt[i], time_one(cand[i].second, 5));
+  int bi = 0;
+  for (int i = 1; i < top; ++i)
+    if (tt[i] < tt[bi]) bi = i;
+  best = cand[bi].second;
   std::lock_guard<std::mutex> g(tune_mu_);
   tuned_[key] = best;
   return best;

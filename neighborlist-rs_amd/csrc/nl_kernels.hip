@@ -1176,7 +1176,7 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 // F32: FP32 filter with FP64 recheck in the band (cell-list items).
 // FULL: full lists (count and fill passes).
 template <bool FILL, int NQ, bool F32, bool FULL>
-__global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
+__device__ __forceinline__ void pairs_body(const PairArgs& A) {
   __shared__ WaveLds s_lds[kWaves];
   const int w = uniform_i32(threadIdx.x >> 6);
   const int64_t k = (int64_t)blockIdx.x * kWaves + w;
@@ -1188,6 +1188,20 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
   // the lists do not fit; the host re-runs the fill with exact buffers
   if (FILL && A.bounds[A.k] > A.capacity) return;
   pair_item<FILL, NQ, F32, FULL>(A, u, s_lds[w]);
+}
+
+template <bool FILL, int NQ, bool F32, bool FULL>
+__global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
+  pairs_body<FILL, NQ, F32, FULL>(A);
+}
+
+// The same kernel held to 80 VGPRs (6 waves per SIMD).  Used for the FP64
+// path without the full-list fill, where that fits without scratch and
+// measured a few per cent faster; the other variants would spill.
+template <bool FILL, int NQ, bool F32, bool FULL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_w6(
+    PairArgs A) {
+  pairs_body<FILL, NQ, F32, FULL>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -1459,23 +1473,26 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
 template <int NQ>
 static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
   const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
-#define NL_PAIRS(FILL, F32)                                                \
-  do {                                                                     \
-    if (a.full)                                                            \
-      hipLaunchKernelGGL((k_pairs<FILL, NQ, F32, true>), g, b, 0, st, a);  \
-    else                                                                   \
-      hipLaunchKernelGGL((k_pairs<FILL, NQ, F32, false>), g, b, 0, st, a); \
-  } while (0)
+#define NL_PAIRS(KERNEL, FILL, F32, FULL) \
+  hipLaunchKernelGGL((KERNEL<FILL, NQ, F32, FULL>), g, b, 0, st, a)
   if (a.use_fp32) {
-    if (fill)
-      NL_PAIRS(true, true);
+    if (fill && a.full)
+      NL_PAIRS(k_pairs, true, true, true);
+    else if (fill)
+      NL_PAIRS(k_pairs, true, true, false);
+    else if (a.full)
+      NL_PAIRS(k_pairs, false, true, true);
     else
-      NL_PAIRS(false, true);
+      NL_PAIRS(k_pairs, false, true, false);
   } else {
-    if (fill)
-      NL_PAIRS(true, false);
+    if (fill && a.full)
+      NL_PAIRS(k_pairs, true, false, true);
+    else if (fill)
+      NL_PAIRS(k_pairs_w6, true, false, false);
+    else if (a.full)
+      NL_PAIRS(k_pairs_w6, false, false, true);
     else
-      NL_PAIRS(false, false);
+      NL_PAIRS(k_pairs_w6, false, false, false);
   }
 #undef NL_PAIRS
 }
