@@ -647,8 +647,8 @@ Tuning Engine::tuning_for(const Request& req) {
   // ... then home-group size and stencil slices jointly (they interact: more
   // slices suit larger groups), FP32 filter on and off for the best pair ...
   std::vector<std::pair<double, Tuning>> cand;
-  for (int g : {2, 4, 8, 16})
-    for (int p : {1, 2, 4}) {
+  for (int g : {4, 8, 16, 32})
+    for (int p : {1, 2, 4, 8}) {
       Tuning x = best;
       x.group = g;
       x.parts = p;
