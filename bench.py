@@ -400,7 +400,9 @@ def main():
                 "traffic_source": None if traffic is None else
                 "profiles/k_pairs_fill_c4_ragged_batch.md (rocprofv3 --pmc FETCH_SIZE, "
                 "WRITE_SIZE; copy-calibrated)",
-                "kernel": "k_pairs<fill>", "kernel_ms": round(fill_ms, 4),
+                "kernel": ("k_pairs<fill, fp32 filter>" if "fp32=1" in tuned
+                           else "k_pairs_w6<fill> (FP64 path)"),
+                "kernel_ms": round(fill_ms, 4),
                 "alg_bytes_per_launch": int(alg_bytes)}
 
     if args.only_headline:
