@@ -47,12 +47,19 @@ int nl_search_systems(const double* positions, size_t n, const int64_t* sys_size
  * NL_FULL gives full (directed) lists: every pair of the half list in both
  * directions, (i, j, S) at an even position e and (j, i, -S) at e + 1; a
  * self image (i, i, S) appears once per S (the half list already holds both
- * S and -S).  Per-system ranges and distances cover the full lists. */
+ * S and -S).  Per-system ranges, distances and displacements cover the full
+ * lists.
+ *
+ * NL_DISP records each edge's displacement r_ij = r_j + H S - r_i in FP64,
+ * the vector whose squared norm the search compared with the cutoff
+ * ((w_j - w_i) + H bs on wrapped positions, in the edge's own orientation),
+ * so sqrt((x^2 + y^2) + z^2) of it is the NL_DIST distance exactly. */
 #define NL_DEVICE_POSITIONS 1u /* positions is a device pointer */
 #define NL_LOCAL_INDEX 2u      /* indices local to each system (else offset by its start) */
 #define NL_RANGES 4u           /* per-system edge ranges (nl_result_system_ranges) */
 #define NL_DIST 8u             /* FP64 |r_ij| per edge (nl_result_copy_dist) */
 #define NL_FULL 16u            /* full (directed) lists */
+#define NL_DISP 32u            /* FP64 r_ij per edge (nl_result_copy_disp) */
 int nl_search_ex(const double* positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
                  const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
                  const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
@@ -60,7 +67,7 @@ int nl_search_ex(const double* positions, size_t n, const int64_t* sys_sizes, si
 
 /* A zero-copy DLPack view (DLManagedTensor*, ROCm device) of one array of
  * list `list`: what = 0 edge_index (2, M) int64, 1 shift (M, 3) int32,
- * 2 distance (M,) float64.  The view shares ownership of the result: the
+ * 2 distance (M,) float64, 3 displacement (M, 3) float64.  The view shares ownership of the result: the
  * lists stay valid until nl_result_free and the deleter of every view have
  * run, whichever comes last.  Work on the arrays belongs on the stream given
  * to nl_search_ex; the view's deleter fences that stream. */
@@ -72,6 +79,8 @@ int nl_result_dlpack(const nl_result* r, size_t list, int what, void** managed);
 int nl_result_system_ranges(const nl_result* r, int64_t* ranges, int32_t* modes);
 /* FP64 |r_ij| of list `list` into dist[M] (requires want_dist). */
 int nl_result_copy_dist(const nl_result* r, size_t list, double* dist);
+/* FP64 r_ij of list `list` into disp[M][3] (requires NL_DISP). */
+int nl_result_copy_disp(const nl_result* r, size_t list, double* disp);
 
 /* Stage timing with hipEvents on the engine stream (milliseconds). */
 void nl_set_timing(int on);

@@ -233,6 +233,7 @@ static int search_opts(const double* positions, size_t n, const int64_t* sys_siz
   req.global_index = (options & NL_LOCAL_INDEX) == 0;
   req.want_ranges = (options & NL_RANGES) != 0;
   req.want_dist = (options & NL_DIST) != 0;
+  req.want_disp = (options & NL_DISP) != 0;
   req.full = (options & NL_FULL) != 0;
   req.has_ustream = has_stream;
   req.ustream = (hipStream_t)stream;
@@ -326,6 +327,25 @@ int nl_result_copy_dist(const nl_result* r, size_t list, double* dist) {
   return 0;
 }
 
+int nl_result_copy_disp(const nl_result* r, size_t list, double* disp) {
+  if (!r || list >= (size_t)r->r->k || !r->r->disp) {
+    nl::set_error("no displacements recorded for this result");
+    return 1;
+  }
+  const Result* R = r->r.get();
+  const int64_t m = R->len[list];
+  if (m == 0) return 0;
+  hipStream_t st = R->eng->stream();
+  hipError_t err = hipMemcpyAsync(disp, (const double*)R->disp + 3 * R->base[list], 24 * m,
+                                  hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (err != hipSuccess) {
+    nl::set_error(std::string("HIP error in nl_result_copy_disp: ") + hipGetErrorString(err));
+    return 1;
+  }
+  return 0;
+}
+
 int nl_result_device(const nl_result* r, size_t list, const int64_t** edge_index,
                      const int32_t** shifts) {
   if (!r || list >= (size_t)r->r->k) {
@@ -403,7 +423,7 @@ int nl_result_dlpack(const nl_result* r, size_t list, int what, void** out) {
     return 1;
   }
   *out = nullptr;
-  if (!r || list >= (size_t)r->r->k || what < 0 || what > 2) {
+  if (!r || list >= (size_t)r->r->k || what < 0 || what > 3) {
     nl::set_error("invalid result, list index or array selector");
     return 1;
   }
@@ -414,6 +434,10 @@ int nl_result_dlpack(const nl_result* r, size_t list, int what, void** out) {
   }
   if (what == 2 && !R->dist) {
     nl::set_error("no distances recorded for this result");
+    return 1;
+  }
+  if (what == 3 && !R->disp) {
+    nl::set_error("no displacements recorded for this result");
     return 1;
   }
   const int64_t m = R->len[list], b = R->base[list];
@@ -440,12 +464,20 @@ int nl_result_dlpack(const nl_result* r, size_t list, int what, void** out) {
     v->shape[1] = 3;
     v->strides[0] = 3;
     v->strides[1] = 1;
-  } else {  // distance (M,) float64
+  } else if (what == 2) {  // distance (M,) float64
     t.data = (double*)R->dist + b;
     t.ndim = 1;
     t.dtype = DLDataType_{2, 64, 1};
     v->shape[0] = m;
     v->strides[0] = 1;
+  } else {  // displacement (M, 3) float64
+    t.data = (double*)R->disp + 3 * b;
+    t.ndim = 2;
+    t.dtype = DLDataType_{2, 64, 1};
+    v->shape[0] = m;
+    v->shape[1] = 3;
+    v->strides[0] = 3;
+    v->strides[1] = 1;
   }
   v->m.manager_ctx = v;
   v->m.deleter = dl_release;

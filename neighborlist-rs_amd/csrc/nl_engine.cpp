@@ -54,6 +54,7 @@ Result::~Result() {
   if (edges) eng->dev_release(edges, edges_bytes, fence);
   if (shifts) eng->dev_release(shifts, shifts_bytes, fence);
   if (dist) eng->dev_release(dist, dist_bytes, fence);
+  if (disp) eng->dev_release(disp, disp_bytes, fence);
 }
 
 static int bits_for(int64_t v) {
@@ -307,11 +308,12 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
 }
 
 hipError_t Pipeline::enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
-                                  int64_t capacity, hipStream_t st) const {
+                                  int64_t capacity, hipStream_t st, void* disp) const {
   PairArgs A = pair_args(*this, b);
   A.edges = (int64_t*)edges;
   A.shifts = (int32_t*)shifts;
   A.dist = (double*)dist;
+  A.disp = (double*)disp;
   A.capacity = capacity;
   for (int q0 = 0; q0 < K; q0 += kGroup) {
     A.q_lo = q0;
@@ -423,11 +425,12 @@ void Engine::dev_release(void* p, size_t bytes, std::shared_ptr<Fence> fence) {
   pool_.emplace(bytes, PoolBlock{p, std::move(fence)});
 }
 
-int Engine::alloc_outputs(Result* res, int64_t edges, bool dist) {
+int Engine::alloc_outputs(Result* res, int64_t edges, bool dist, bool disp) {
   if (res->edges) dev_release(res->edges, res->edges_bytes);
   if (res->shifts) dev_release(res->shifts, res->shifts_bytes);
   if (res->dist) dev_release(res->dist, res->dist_bytes);
-  res->edges = res->shifts = res->dist = nullptr;
+  if (res->disp) dev_release(res->disp, res->disp_bytes);
+  res->edges = res->shifts = res->dist = res->disp = nullptr;
   const int64_t m = std::max<int64_t>(edges, 1);
   res->edges_bytes = sizeof(int64_t) * 2 * m;
   res->shifts_bytes = sizeof(int32_t) * 3 * m;
@@ -437,7 +440,11 @@ int Engine::alloc_outputs(Result* res, int64_t edges, bool dist) {
     res->dist_bytes = sizeof(double) * m;
     res->dist = dev_alloc(res->dist_bytes);
   }
-  if (!res->edges || !res->shifts || (dist && !res->dist)) {
+  if (disp) {
+    res->disp_bytes = sizeof(double) * 3 * m;
+    res->disp = dev_alloc(res->disp_bytes);
+  }
+  if (!res->edges || !res->shifts || (dist && !res->dist) || (disp && !res->disp)) {
     set_error("out of device memory for the edge lists");
     return 1;
   }
@@ -514,12 +521,12 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
   const int64_t cap = have_est ? last_total_ + last_total_ / 8 + 4096 : -1;
   auto launch_fill = [&](int64_t capacity) -> int {
     if (timing_) NL_CHECK(hipEventRecord(ev_[3], st));
-    NL_CHECK(P.enqueue_fill(B, res->edges, res->shifts, res->dist, capacity, st));
+    NL_CHECK(P.enqueue_fill(B, res->edges, res->shifts, res->dist, capacity, st, res->disp));
     if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
     return 0;
   };
   if (have_est) {
-    if (alloc_outputs(res.get(), cap, req.want_dist)) return 1;
+    if (alloc_outputs(res.get(), cap, req.want_dist, req.want_disp)) return 1;
     if (launch_fill(cap)) return 1;
   }
   NL_CHECK(P.enqueue_bounds(B, pin, st));
@@ -528,7 +535,7 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
   const int64_t total = hb[K];
   res->total = total;
   if (!have_est || total > cap) {
-    if (alloc_outputs(res.get(), total, req.want_dist)) return 1;
+    if (alloc_outputs(res.get(), total, req.want_dist, req.want_disp)) return 1;
     if (total > 0 && launch_fill(total)) return 1;
   }
   for (int q = 0; q < 4; ++q) last_sig_[q] = sig[q];
@@ -612,6 +619,7 @@ Tuning Engine::tuning_for(const Request& req) {
   Request probe = req;
   probe.want_ranges = false;
   probe.want_dist = false;
+  probe.want_disp = false;
   // device time of the search stages (hipEvents around the launches), best
   // of `reps` runs
   auto time_one = [&](const Tuning& x, int reps) -> double {

@@ -51,6 +51,7 @@ struct Request {
   bool global_index = true;
   bool want_ranges = false;     // per-system edge ranges
   bool want_dist = false;       // FP64 |r_ij| per edge
+  bool want_disp = false;       // FP64 r_ij = r_j + H S - r_i per edge
   bool full = false;            // full lists: (i, j, S) and (j, i, -S) for every pair
   // the caller's stream: device-resident positions are read after the work
   // already queued on it, and released outputs are reused only after it
@@ -106,7 +107,7 @@ class Pipeline {
   hipError_t enqueue_upload(const Buffers& b, const char* pinned, hipStream_t st) const;
   hipError_t enqueue_build_count(const Buffers& b, hipStream_t st) const;
   hipError_t enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
-                          int64_t capacity, hipStream_t st) const;
+                          int64_t capacity, hipStream_t st, void* disp = nullptr) const;
   hipError_t enqueue_bounds(const Buffers& b, char* pinned, hipStream_t st) const;
   const int64_t* host_bounds(const char* pinned) const;
   int rank_of(int list) const;  // input list -> sorted rank
@@ -156,6 +157,8 @@ struct Result {
   size_t shifts_bytes = 0;
   void* dist = nullptr;        // double [capacity] (optional)
   size_t dist_bytes = 0;
+  void* disp = nullptr;        // double [3 * capacity] (optional)
+  size_t disp_bytes = 0;
   std::vector<int64_t> ranges; // [s][list(input order)][2] global edge ids (optional)
   std::vector<int32_t> modes;  // per system (optional, with ranges)
   Tuning tune;
@@ -192,7 +195,7 @@ class Engine {
   void set_timing(bool on) { timing_ = on; }
   std::mutex& mutex() { return mu_; }
   int fail(hipError_t e, const char* where);
-  int alloc_outputs(Result* res, int64_t edges, bool dist);
+  int alloc_outputs(Result* res, int64_t edges, bool dist, bool disp = false);
   std::string tune_report();
   void clear_tuning();
 

@@ -29,6 +29,7 @@ struct PairArgs {
   int64_t* edges;            // list q: sources at base_q + e, targets at base_q + M_q + e
   int32_t* shifts;           // list q: 3 per edge at 3 e
   double* dist;              // optional per edge: d^2 from the fill pass, then |r_ij|
+  double* disp;              // optional per edge: r_ij = r_j + H S - r_i, 3 doubles
   int64_t n;                 // work items
   int64_t capacity;          // edges the output buffers hold (fill pass)
   int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index;

@@ -629,9 +629,13 @@ using Acc = typename std::conditional<FILL, int64_t, uint32_t>::type;
 // (edge, distance) and 8-byte (shift) stores.  A self image (i, i, S) is
 // emitted once by the half stencil and mirrored to (i, i, -S), which the
 // half list writes as a second emit instead.
-template <bool FILL, int NQ, bool FULL = false>
+//
+// DISP: (ex, ey, ez), the edge's displacement r_ij in its own orientation,
+// is stored to A.disp (the mirror gets its negation).
+template <bool FILL, int NQ, bool FULL = false, bool DISP = false>
 __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
-                                 int64_t gj, int sx, int sy, int sz, double d2, Acc<FILL>* cur) {
+                                 int64_t gj, int sx, int sy, int sz, double d2, double ex,
+                                 double ey, double ez, Acc<FILL>* cur) {
   if (!FILL) {
 #pragma unroll
     for (int g = 0; g < NQ; ++g) cur[g] += (mem >> g) & 1u;
@@ -659,6 +663,12 @@ __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, 
         q[1] = make_int2(sz, -sx);
         q[2] = make_int2(-sy, -sz);
         if (A.dist) reinterpret_cast<double2*>(A.dist + cur[g])[lb] = make_double2(d2, d2);
+        if (DISP) {
+          double2* r = reinterpret_cast<double2*>(A.disp + 3 * cur[g]) + 3u * lb;
+          r[0] = make_double2(ex, ey);
+          r[1] = make_double2(ez, -ex);
+          r[2] = make_double2(-ey, -ez);
+        }
       } else {
         src[lb] = gi;
         dst[lb] = gj;
@@ -667,13 +677,19 @@ __device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, 
         q[1] = sy;
         q[2] = sz;
         if (A.dist) A.dist[cur[g] + lb] = d2;  // squared here; square roots in a pass of their own
+        if (DISP) {
+          double* r = A.disp + 3 * cur[g] + 3u * lb;
+          r[0] = ex;
+          r[1] = ey;
+          r[2] = ez;
+        }
       }
     }
     cur[g] += (FULL ? 2 : 1) * (int64_t)__popcll(m);
   }
 }
 
-template <bool FILL, int NQ, bool FULL>
+template <bool FILL, int NQ, bool FULL, bool DISP>
 __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
                                 int mode, Acc<FILL>* cur) {
   const int lane = lane_id();
@@ -724,7 +740,7 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
     const uint32_t mem = valid ? members64_rt(d2, C) : 0u;
     if (FILL && __ballot(mem != 0) == 0) continue;
     const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
-    emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
+    emit_step<FILL, NQ, FULL, DISP>(A, C, mem, gi, gj, sx, sy, sz, d2, dx, dy, dz, cur);
   }
 }
 
@@ -753,7 +769,7 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 // orientation (lower original index first, S = s_j - s_i + bs from it,
 // search.rs:695-697); a self image (same atom, bs != 0) is written for both
 // S and -S, as the reference emits both (search.rs:549).
-template <bool FILL, int NQ, bool F32, bool FULL>
+template <bool FILL, int NQ, bool F32, bool FULL, bool DISP>
 __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
                           int part, WaveLds& L, Acc<FILL>* cur) {
   const int lane = lane_id();
@@ -1001,7 +1017,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                 bool unsure = false;
                 mem = members32<DISJ, NQ>(e2, C, &unsure);
                 mem = cand ? mem : 0u;
-                const bool need64 = cand && (unsure || (A.dist != nullptr && mem != 0));
+                const bool need64 =
+                    cand && (unsure || ((DISP || A.dist != nullptr) && mem != 0));
                 if (__ballot(need64)) {
                   const Pos64 h64 = A.spos64[hb + i];  // uniform address
                   const double xi64 = h64.x, yi64 = h64.y, zi64 = h64.z;
@@ -1040,12 +1057,28 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                   gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
                 }
               }
-              emit_step<FILL, NQ, FULL>(A, C, mem, gi, gj, sx, sy, sz, d2, cur);
+              // displacement in the edge's orientation (lower original index
+            // first), recomputed from the operands of d^2 only when asked for
+            double ex = 0.0, ey = 0.0, ez = 0.0;
+            if (DISP && mem != 0) {
+              const Pos64 cc = F32 ? A.spos64[cjs] : c;
+              const Pos64 hh = F32 ? A.spos64[hb + i] : h64;
+              ex = (cc.x - hh.x) + cox;
+              ey = (cc.y - hh.y) + coy;
+              ez = (cc.z - hh.z) + coz;
+              if (oj < oi) {
+                ex = -ex;
+                ey = -ey;
+                ez = -ez;
+              }
+            }
+            emit_step<FILL, NQ, FULL, DISP>(A, C, mem, gi, gj, sx, sy, sz, d2, ex, ey, ez, cur);
               // the self image's -S copy (full lists get it from the mirror;
               // their count pass counts each half-stencil hit once, doubled
               // when the item ends)
               if (!FULL && (!FILL || __ballot(self_img)))
-                emit_step<FILL, NQ>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz, d2, cur);
+                emit_step<FILL, NQ, false, DISP>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz,
+                                                 d2, -ex, -ey, -ez, cur);
             }
           };
           if (C.disjoint)
@@ -1067,7 +1100,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
 // takes the cell-list path; slots that do not start a group are empty).
 // Edges of one system are thus contiguous and systems come out in input
 // order, as batch.rs:135-144 concatenates them.
-template <bool FILL, int NQ, bool F32, bool FULL>
+template <bool FILL, int NQ, bool F32, bool FULL, bool DISP>
 __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds& L) {
   const int s = uniform_i32(find_sys(A.item_base, A.n_sys, u));
   const SysDesc* d = A.sys + s;
@@ -1098,11 +1131,12 @@ __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds&
   }
   const int n_atoms = d->n_atoms;
   if (local < n_atoms) {
-    if (mode != kCellList) unit_bruteforce<FILL, NQ, FULL>(A, C, d, d->atom_start + local, mode, cur);
+    if (mode != kCellList)
+      unit_bruteforce<FILL, NQ, FULL, DISP>(A, C, d, d->atom_start + local, mode, cur);
   } else if (mode == kCellList) {
     const int64_t r = local - n_atoms;
     const int64_t p = d->atom_start + r / A.parts;
-    unit_cell<FILL, NQ, F32, FULL>(A, C, d, p, (int)(r % A.parts), L, cur);
+    unit_cell<FILL, NQ, F32, FULL, DISP>(A, C, d, p, (int)(r % A.parts), L, cur);
   }
   if (!FILL) {
 #pragma unroll
@@ -1174,8 +1208,9 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 // One wave per listed work item (a loop over several items per wave keeps
 // the per-item state live across iterations and costs about 40 VGPRs).
 // F32: FP32 filter with FP64 recheck in the band (cell-list items).
-// FULL: full lists (count and fill passes).
-template <bool FILL, int NQ, bool F32, bool FULL>
+// FULL: full lists (count and fill passes).  DISP: the fill pass also
+// stores displacements.
+template <bool FILL, int NQ, bool F32, bool FULL, bool DISP>
 __device__ __forceinline__ void pairs_body(const PairArgs& A) {
   __shared__ WaveLds s_lds[kWaves];
   const int w = uniform_i32(threadIdx.x >> 6);
@@ -1187,12 +1222,12 @@ __device__ __forceinline__ void pairs_body(const PairArgs& A) {
   // fill pass launched before the host saw the counts: write nothing when
   // the lists do not fit; the host re-runs the fill with exact buffers
   if (FILL && A.bounds[A.k] > A.capacity) return;
-  pair_item<FILL, NQ, F32, FULL>(A, u, s_lds[w]);
+  pair_item<FILL, NQ, F32, FULL, DISP>(A, u, s_lds[w]);
 }
 
-template <bool FILL, int NQ, bool F32, bool FULL>
+template <bool FILL, int NQ, bool F32, bool FULL, bool DISP = false>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
-  pairs_body<FILL, NQ, F32, FULL>(A);
+  pairs_body<FILL, NQ, F32, FULL, DISP>(A);
 }
 
 // The same kernel held to 80 VGPRs (6 waves per SIMD).  Used for the FP64
@@ -1201,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
 template <bool FILL, int NQ, bool F32, bool FULL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_w6(
     PairArgs A) {
-  pairs_body<FILL, NQ, F32, FULL>(A);
+  pairs_body<FILL, NQ, F32, FULL, false>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -1475,7 +1510,21 @@ static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
   const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
 #define NL_PAIRS(KERNEL, FILL, F32, FULL) \
   hipLaunchKernelGGL((KERNEL<FILL, NQ, F32, FULL>), g, b, 0, st, a)
-  if (a.use_fp32) {
+#define NL_PAIRS_DISP(F32, FULL) \
+  hipLaunchKernelGGL((k_pairs<true, NQ, F32, FULL, true>), g, b, 0, st, a)
+  if (fill && a.disp) {  // fill pass that also stores displacements
+    if (a.use_fp32) {
+      if (a.full)
+        NL_PAIRS_DISP(true, true);
+      else
+        NL_PAIRS_DISP(true, false);
+    } else {
+      if (a.full)
+        NL_PAIRS_DISP(false, true);
+      else
+        NL_PAIRS_DISP(false, false);
+    }
+  } else if (a.use_fp32) {
     if (fill && a.full)
       NL_PAIRS(k_pairs, true, true, true);
     else if (fill)
@@ -1494,6 +1543,7 @@ static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
     else
       NL_PAIRS(k_pairs_w6, false, false, false);
   }
+#undef NL_PAIRS_DISP
 #undef NL_PAIRS
 }
 

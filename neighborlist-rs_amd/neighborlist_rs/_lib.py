@@ -59,6 +59,7 @@ _SIGS = {
     "nl_result_dlpack": (C.c_int, [res_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     "nl_result_system_ranges": (C.c_int, [res_p, i64p, i32p]),
     "nl_result_copy_dist": (C.c_int, [res_p, C.c_size_t, dp]),
+    "nl_result_copy_disp": (C.c_int, [res_p, C.c_size_t, dp]),
     "nl_set_timing": (None, [C.c_int]),
     "nl_last_timing": (C.c_int, [fp, fp, fp, fp, i64p, i64p]),
     "nl_set_cell_div": (None, [C.c_int]),

@@ -8,7 +8,7 @@ device and returning the lists as torch tensors on that device, with no copy
 through host memory: the tensors are DLPack views of the engine's own output
 blocks (nl_result_dlpack), which stay alive as long as any view does.
 
-Two options go beyond the reference (both keyword-only):
+Three options go beyond the reference (all keyword-only):
 
 * ``full=True``: full (directed) lists, every pair of the half list in both
   directions, (i, j, S) at an even position and (j, i, -S) right after it; a
@@ -17,6 +17,8 @@ Two options go beyond the reference (both keyword-only):
 * ``distances=True``: adds ``"distance"``, the float64 length
   |r_j + S H^T - r_i| of every edge, from the same expression the search
   compares with the cutoff.
+* ``displacements=True``: adds ``"displacement"``, the float64 vector
+  r_j + S H^T - r_i of every edge (M, 3), the one whose norm is the distance.
 
 Import torch before this package: torch carries its own HIP runtime, and
 the engine library binds to it only if it is loaded first (otherwise the
@@ -42,6 +44,7 @@ NL_LOCAL_INDEX = 2
 NL_RANGES = 4
 NL_DIST = 8
 NL_FULL = 16
+NL_DISP = 32
 
 _DLTENSOR = b"dltensor"  # capsule name; kept alive by the module
 _capsule_new = C.pythonapi.PyCapsule_New
@@ -91,37 +94,41 @@ def _positions_t(positions):
 
 def _view(res, q, what):
     """Zero-copy torch view of one array of list q (0 edge_index, 1 shift,
-    2 distance)."""
+    2 distance, 3 displacement)."""
     torch = _torch()
     p = C.c_void_p()
     _check(lib.nl_result_dlpack(res.ptr, q, what, C.byref(p)))
     return torch.utils.dlpack.from_dlpack(_capsule_new(p, _DLTENSOR, None))
 
 
-def _empty_t(device, distances):
+def _empty_t(device, distances, displacements=False):
     torch = _torch()
     out = {"edge_index": torch.zeros((2, 0), dtype=torch.int64, device=device),
            "shift": torch.zeros((0, 3), dtype=torch.int32, device=device)}
     if distances:
         out["distance"] = torch.zeros((0,), dtype=torch.float64, device=device)
+    if displacements:
+        out["displacement"] = torch.zeros((0, 3), dtype=torch.float64, device=device)
     return out
 
 
-def _lists(res, k, device, distances):
+def _lists(res, k, device, distances, displacements):
     out = []
     for q in range(k):
         m = int(lib.nl_result_num_edges(res.ptr, q))
         if m == 0:
-            out.append(_empty_t(device, distances))
+            out.append(_empty_t(device, distances, displacements))
             continue
         d = {"edge_index": _view(res, q, 0), "shift": _view(res, q, 1)}
         if distances:
             d["distance"] = _view(res, q, 2)
+        if displacements:
+            d["displacement"] = _view(res, q, 3)
         out.append(d)
     return out
 
 
-def _search(pos, sizes, H, has, pb, cutoffs, disjoint, multi, full, distances):
+def _search(pos, sizes, H, has, pb, cutoffs, disjoint, multi, full, distances, displacements):
     from . import _Result
     torch = _torch()
     dev = pos.device
@@ -129,7 +136,8 @@ def _search(pos, sizes, H, has, pb, cutoffs, disjoint, multi, full, distances):
     stream = torch.cuda.current_stream(dev).cuda_stream
     sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.int64))
     c = np.ascontiguousarray(np.asarray(cutoffs, dtype=np.float64))
-    opts = NL_DEVICE_POSITIONS | (NL_FULL if full else 0) | (NL_DIST if distances else 0)
+    opts = (NL_DEVICE_POSITIONS | (NL_FULL if full else 0) | (NL_DIST if distances else 0) |
+            (NL_DISP if displacements else 0))
     res = _Result()
     _check(lib.nl_search_ex(
         C.c_void_p(pos.data_ptr()), pos.shape[0], sz.ctypes.data_as(_lib.i64p), sz.size,
@@ -137,7 +145,7 @@ def _search(pos, sizes, H, has, pb, cutoffs, disjoint, multi, full, distances):
         None if has is None else has.ctypes.data_as(_lib.u8p),
         None if pb is None else pb.ctypes.data_as(_lib.u8p), c.ctypes.data_as(_lib.dp), c.size,
         int(bool(disjoint)), int(bool(multi)), opts, C.c_void_p(stream), C.byref(res.ptr)))
-    return _lists(res, c.size, dev, distances)
+    return _lists(res, c.size, dev, distances, displacements)
 
 
 def _single_cell(cell):
@@ -188,7 +196,8 @@ def _check_cutoffs(cutoffs, labels):
     return cutoffs, labels
 
 
-def build_neighborlists_torch(cell, positions, cutoff, *, full=False, distances=False):
+def build_neighborlists_torch(cell, positions, cutoff, *, full=False, distances=False,
+                              displacements=False):
     """build_neighborlists (src/python_api.rs:139-186) on device tensors."""
     cutoff = float(cutoff)
     if not np.isfinite(cutoff) or cutoff <= 0.0:
@@ -196,15 +205,16 @@ def build_neighborlists_torch(cell, positions, cutoff, *, full=False, distances=
     pos = _positions_t(positions)
     n = pos.shape[0]
     if n == 0:
-        return _empty_t(pos.device, distances)
+        return _empty_t(pos.device, distances, displacements)
     H, has, pb = _single_cell(cell)
-    out = _search(pos, [n], H, has, pb, [cutoff], False, False, full, distances)
+    out = _search(pos, [n], H, has, pb, [cutoff], False, False, full, distances,
+                  displacements)
     _log("build_neighborlists_torch", n)
     return out[0]
 
 
 def build_neighborlists_multi_torch(cell, positions, cutoffs, labels=None, disjoint=False, *,
-                                    full=False, distances=False):
+                                    full=False, distances=False, displacements=False):
     """build_neighborlists_multi (src/python_api.rs:235-309) on device tensors."""
     pos = _positions_t(positions)
     n = pos.shape[0]
@@ -212,15 +222,17 @@ def build_neighborlists_multi_torch(cell, positions, cutoffs, labels=None, disjo
     if len(cutoffs) == 0:
         return {}
     if n == 0:
-        return _keyed([_empty_t(pos.device, distances) for _ in cutoffs], labels, len(cutoffs))
+        return _keyed([_empty_t(pos.device, distances, displacements) for _ in cutoffs], labels,
+                      len(cutoffs))
     H, has, pb = _single_cell(cell)
-    out = _search(pos, [n], H, has, pb, cutoffs, disjoint, True, full, distances)
+    out = _search(pos, [n], H, has, pb, cutoffs, disjoint, True, full, distances,
+                  displacements)
     _log("build_neighborlists_multi_torch", n)
     return _keyed(out, labels, len(cutoffs))
 
 
 def build_neighborlists_batch_torch(positions, batch, cells=None, cutoff=5.0, *, full=False,
-                                    distances=False):
+                                    distances=False, displacements=False):
     """build_neighborlists_batch (src/python_api.rs:315-423) on device
     tensors; indices are offset by each system's start, as there."""
     cutoff = float(cutoff)
@@ -230,16 +242,17 @@ def build_neighborlists_batch_torch(positions, batch, cells=None, cutoff=5.0, *,
     n = pos.shape[0]
     sizes = _batch_sizes(batch, n)
     if n == 0:
-        return _empty_t(pos.device, distances)
+        return _empty_t(pos.device, distances, displacements)
     _, H, has, pb = _batch_cells_t(cells, sizes.size)
-    out = _search(pos, sizes, H, has, pb, [cutoff], False, False, full, distances)
+    out = _search(pos, sizes, H, has, pb, [cutoff], False, False, full, distances,
+                  displacements)
     _log("build_neighborlists_batch_torch", n)
     return out[0]
 
 
 def build_neighborlists_batch_multi_torch(positions, batch, cells=None, cutoffs=[5.0],
                                           labels=None, disjoint=False, *, full=False,
-                                          distances=False):
+                                          distances=False, displacements=False):
     """build_neighborlists_batch_multi (src/python_api.rs:425-546) on device
     tensors."""
     pos = _positions_t(positions)
@@ -249,6 +262,7 @@ def build_neighborlists_batch_multi_torch(positions, batch, cells=None, cutoffs=
     if n == 0 or len(cutoffs) == 0:
         return {}
     _, H, has, pb = _batch_cells_t(cells, sizes.size)
-    out = _search(pos, sizes, H, has, pb, cutoffs, disjoint, True, full, distances)
+    out = _search(pos, sizes, H, has, pb, cutoffs, disjoint, True, full, distances,
+                  displacements)
     _log("build_neighborlists_batch_multi_torch", n)
     return _keyed(out, labels, len(cutoffs))

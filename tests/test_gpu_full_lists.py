@@ -304,3 +304,126 @@ def test_torch_errors_and_empty(nr):
     r = nr.build_neighborlists_torch(None, torch.zeros((1, 3), dtype=torch.float64).cuda(), 2.0,
                                      full=True)
     assert tuple(r["edge_index"].shape) == (2, 0)
+
+
+# ---------------------------------------------------------------------------
+# displacements r_ij = r_j + H S - r_i (NL_DISP), half and full lists
+
+NL_DISP = 32
+
+
+def _check_disp(pos, cell, got, what):
+    ei, ej, s, d, disp = got
+    H = np.zeros((3, 3)) if cell is None else np.asarray(cell[0])
+    want = pos[ej] + s.astype(np.float64) @ H.T - pos[ei]
+    np.testing.assert_allclose(disp, want, rtol=0, atol=1e-12, err_msg=what)
+    # the vector whose norm the search compared: its norm is the distance
+    n = np.sqrt((disp[:, 0] * disp[:, 0] + disp[:, 1] * disp[:, 1]) + disp[:, 2] * disp[:, 2])
+    assert np.all(np.abs(n - d) <= 2 * np.spacing(np.maximum(n, d))), what
+
+
+def _ex_disp(pos, sizes, cells, cutoffs, disjoint=False, multi=False, opts=0):
+    from neighborlist_rs import _lib
+    out = _ex(pos, sizes, cells, cutoffs, disjoint, multi, opts | NL_DIST)
+    # second call for the displacements (the helper copies edges and distances)
+    lib = _lib.lib
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    sz = np.ascontiguousarray(sizes, dtype=np.int64)
+    S = sz.size
+    Hs = np.zeros((S, 9))
+    has = np.zeros(S, np.uint8)
+    pb = np.ones((S, 3), np.uint8)
+    for k, c in enumerate(cells):
+        if c is not None:
+            Hs[k] = np.asarray(c[0], dtype=np.float64).reshape(9)
+            has[k] = 1
+            pb[k] = [1 if v else 0 for v in c[1]]
+    cut = np.ascontiguousarray(cutoffs, dtype=np.float64)
+    r = C.c_void_p()
+    assert lib.nl_search_ex(p.ctypes.data_as(C.c_void_p), p.shape[0], sz.ctypes.data_as(_lib.i64p),
+                            S, Hs.ctypes.data_as(_lib.dp), has.ctypes.data_as(_lib.u8p),
+                            pb.ctypes.data_as(_lib.u8p), cut.ctypes.data_as(_lib.dp), cut.size,
+                            int(disjoint), int(multi), opts | NL_DIST | NL_DISP, None,
+                            C.byref(r)) == 0, _lib.last_error()
+    res = []
+    try:
+        for q in range(cut.size):
+            m = int(lib.nl_result_num_edges(r, q))
+            ei, ej = np.empty(m, np.int64), np.empty(m, np.int64)
+            sh = np.empty((m, 3), np.int32)
+            d, v = np.empty(m), np.empty((m, 3))
+            if m:
+                assert lib.nl_result_copy(r, q, ei.ctypes.data_as(_lib.i64p),
+                                          ej.ctypes.data_as(_lib.i64p),
+                                          sh.ctypes.data_as(_lib.i32p)) == 0
+                assert lib.nl_result_copy_dist(r, q, d.ctypes.data_as(_lib.dp)) == 0
+                assert lib.nl_result_copy_disp(r, q, v.ctypes.data_as(_lib.dp)) == 0
+            # the lists themselves do not change when displacements are asked for
+            np.testing.assert_array_equal(ei, out[q][0])
+            np.testing.assert_array_equal(ej, out[q][1])
+            np.testing.assert_array_equal(sh, out[q][2])
+            np.testing.assert_array_equal(d, out[q][3])
+            res.append((ei, ej, sh, d, v))
+    finally:
+        lib.nl_result_free(r)
+    return res
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_displacements_half_and_full(case):
+    name, pos, cell, rc = _cases()[case]
+    _print_case("displacements", name)
+    half = _ex_disp(pos, [pos.shape[0]], [cell], [rc])[0]
+    full = _ex_disp(pos, [pos.shape[0]], [cell], [rc], opts=NL_FULL)[0]
+    _check_disp(pos, cell, half, name + " half")
+    _check_disp(pos, cell, full, name + " full")
+    np.testing.assert_array_equal(full[4][1::2], -full[4][0::2])
+    if int((half[0] == half[1]).sum()) == 0:
+        np.testing.assert_array_equal(full[4][0::2], half[4])
+
+
+def test_displacements_fp32_filter_and_batch():
+    from neighborlist_rs import _lib
+    from neighborlist_rs.workloads import c4_ragged_batch
+    lib = _lib.lib
+    pos, sizes, cells_list, cutoffs = c4_ragged_batch(seed=6, n_systems=20, nmax=800)
+    cells = [None if c is None else (np.asarray(c), [True] * 3) for c in cells_list]
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    was = lib.nl_get_autotune()
+    lib.nl_set_autotune(0)
+    try:
+        got = {}
+        for f32 in (0, 1):
+            lib.nl_set_fp32_filter(f32)
+            _print_case("displacements batch fp32", f32)
+            got[f32] = _ex_disp(pos, sizes, cells, cutoffs, False, True)
+    finally:
+        lib.nl_set_fp32_filter(0)
+        lib.nl_set_autotune(was)
+    for q in range(len(cutoffs)):
+        # FP32 filter with FP64 recheck: the same lists and displacements, bit for bit
+        for a, b in zip(got[0][q], got[1][q]):
+            np.testing.assert_array_equal(a, b)
+        ei, ej, s, d, v = got[0][q]
+        for k in range(len(sizes)):
+            sel = (ei >= starts[k]) & (ei < starts[k + 1])
+            _check_disp(pos, cells[k], (ei[sel], ej[sel], s[sel], d[sel], v[sel]),
+                        f"list {q} system {k}")
+
+
+def test_torch_displacements(nr):
+    import torch
+    rng = np.random.default_rng(12)
+    H = np.array([[12.0, 1.0, 0.5], [0.0, 11.0, 1.0], [0.0, 0.0, 13.0]])
+    pos = rng.uniform(0, 1, (900, 3)) @ H.T
+    cell = nr.PyCell(H.tolist(), [True] * 3)
+    _print_case("torch displacements")
+    g = nr.build_neighborlists_torch(cell, torch.from_numpy(pos).cuda(), 5.0, full=True,
+                                     distances=True, displacements=True)
+    v = g["displacement"]
+    assert v.dtype == torch.float64 and tuple(v.shape) == (g["shift"].shape[0], 3)
+    e, s = _np(g["edge_index"]), _np(g["shift"])
+    _check_disp(pos, (H, [True] * 3), (e[0], e[1], s, _np(g["distance"]), _np(v)), "torch")
+    r = nr.build_neighborlists_torch(None, torch.zeros((1, 3), dtype=torch.float64).cuda(), 2.0,
+                                     displacements=True)
+    assert tuple(r["displacement"].shape) == (0, 3)
