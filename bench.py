@@ -437,6 +437,26 @@ def main():
                   "timing": "hipEvents around the device stages of the eager path"}
     log(f"full lists: {full_edges} edges, fill {full_fill:.3f} ms")
 
+    # half lists with FP64 distances and displacements (28 + 8 + 24 B per edge)
+    NL_DIST, NL_DISP = 8, 32
+    geo = NL_DEVICE_POSITIONS | NL_DIST | NL_DISP
+    for _ in range(2):
+        run_ex(lib, L, dpos.ptr, n, sizes, H9, has, pbc, cut, True, geo)
+    gfill, gtot = [], []
+    for _ in range(max(5, args.steps // 2)):
+        geo_edges = run_ex(lib, L, dpos.ptr, n, sizes, H9, has, pbc, cut, True, geo)
+        b, c, f, t, _ = timing(lib)
+        gfill.append(f)
+        gtot.append(t)
+    geo_fill = float(np.mean(gfill))
+    geometry = {"edges_per_step": int(geo_edges),
+                "fill_ms": round(geo_fill, 4),
+                "device_ms": round(float(np.mean(gtot)), 4),
+                "fill_alg_GBps": round((60.0 * geo_edges + 24.0 * n) / (geo_fill * 1e-3) / 1e9, 2),
+                "bytes_per_edge": 60,
+                "timing": "hipEvents around the device stages (fill includes the square-root pass)"}
+    log(f"lists + distances + displacements: fill {geo_fill:.3f} ms")
+
     # ---- plain path: one call per system (same inputs, device-resident) --
     starts = np.concatenate([[0], np.cumsum(sizes)])
     one = np.ones(1, np.int64)
@@ -543,6 +563,7 @@ def main():
                              else round(c3_cpu_ms / c3_ms, 2)},
         "precision_c3_rc6": precision,
         "full_lists_c4": full_lists,
+        "distances_displacements_c4": geometry,
         "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
