@@ -106,6 +106,9 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   double rc_max = NAN;  // NaN-ignoring fold (single.rs:106)
   for (double c : req.cutoffs) rc_max = std::isnan(rc_max) ? c : std::fmax(rc_max, c);
 
+  tune.group = std::max(1, std::min(64, tune.group));
+  tune.parts = std::max(1, std::min(16, tune.parts));
+  tune.cell_div = std::max(1, std::min(4, tune.cell_div));
   // per-system descriptors (static part), bin reservation
   desc.assign(S, SysDesc{});
   bin_total = 0;
@@ -117,6 +120,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
     d.has_cell = req.sys[s].has_cell;
     d.rc_max = per_sys ? req.sys_cutoff[s] : rc_max;
     for (int k = 0; k < 3; ++k) d.pbc[k] = req.sys[s].has_cell ? (req.sys[s].pbc[k] != 0) : 0;
+    int64_t cap = std::min<int64_t>(int64_t(1) << 22, 2 * (int64_t)d.n_atoms + 64);
     if (d.has_cell) {
       std::memcpy(d.h, req.sys[s].h, sizeof d.h);
       double inv[9];
@@ -124,8 +128,20 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
         set_error("Cell matrix is not invertible");  // cell.rs:6
         return 1;
       }
+      // the grid k_plan will choose for this cell (fewer key bits, often
+      // one radix pass less); should it come out larger on the device,
+      // k_plan shrinks it to the reservation, which any covering grid allows
+      double w[3];
+      widths3(inv, w);
+      const double edge = d.rc_max / (double)tune.cell_div;
+      int64_t prod = 1;
+      for (int k = 0; k < 3; ++k) {
+        const double v = std::floor(w[k] / edge);
+        prod *= v < 1.0 ? 1 : (v > 4096.0 ? 4096 : (int64_t)v);
+        if (prod >= cap) break;
+      }
+      cap = std::min(cap, prod);
     }
-    const int64_t cap = std::min<int64_t>(int64_t(1) << 22, 2 * (int64_t)d.n_atoms + 64);
     d.bin_cap = (int32_t)cap;
     d.bin_base = bin_total;
     bin_total += cap;
@@ -143,9 +159,6 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   rtiles = radix_tiles(n);
   // work items: per system its atoms (brute-force rows), then
   // (slot, stencil slice) pairs (cell-list home groups)
-  tune.group = std::max(1, std::min(64, tune.group));
-  tune.parts = std::max(1, std::min(16, tune.parts));
-  tune.cell_div = std::max(1, std::min(4, tune.cell_div));
   n_items = n * (1 + tune.parts);
   starts = req.starts;
   item_base.resize(S + 1);
@@ -655,7 +668,7 @@ Tuning Engine::tuning_for(const Request& req) {
   // ... then home-group size and stencil slices jointly (they interact: more
   // slices suit larger groups), FP32 filter on and off for the best pair ...
   std::vector<std::pair<double, Tuning>> cand;
-  for (int g : {4, 8, 16, 32})
+  for (int g : {2, 4, 8, 16, 32})
     for (int p : {1, 2, 4, 8}) {
       Tuning x = best;
       x.group = g;
