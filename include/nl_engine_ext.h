@@ -171,6 +171,42 @@ int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t*
 /* waits for every submitted request, stops the slot threads, frees the buffers */
 void nl_server_free(nl_server* s);
 
+/* ---- skin lists for molecular dynamics -------------------------------------
+ * The dynamic manager of the reference's specification (spec.md:41-48,
+ * 209-222; never implemented there): lists built at r_list = rc_q + skin_q
+ * from the positions of the last build (the reference positions) and kept
+ * while no atom has moved more than min_q(skin_q) / 2 since then, so every
+ * pair now within rc_q is in list q.  The lists are half (or, with NL_FULL,
+ * full) lists at r_list; a consumer applies rc itself.
+ *
+ * nl_skin_create copies the device positions and builds once (a
+ * graph-captured plan).  nl_skin_update measures the largest displacement
+ * from the reference positions on the device and rebuilds when it exceeds
+ * the bound (or is NaN), or when `cells` (row-major H per system, NULL =
+ * unchanged) differs from the current cells; *rebuilt says whether it did.
+ * nl_skin_check measures without changing anything.  Device list pointers
+ * (nl_skin_device, same layout as nl_result_device) stay valid until the
+ * next update that rebuilds.  options: NL_LOCAL_INDEX, NL_FULL.  stream:
+ * the caller's stream, as for nl_search_ex.  Positions must keep one
+ * continuous trajectory (no re-wrapping into the cell between updates, or a
+ * wrapped atom counts as a large displacement and forces a rebuild). */
+typedef struct nl_skin nl_skin;
+int nl_skin_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                   const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                   const double* cutoffs, const double* skins, size_t n_cutoffs, int disjoint,
+                   int multi, unsigned options, void* stream, nl_skin** out);
+int nl_skin_update(nl_skin* s, const double* d_positions, const double* cells, void* stream,
+                   int* rebuilt);
+int nl_skin_check(nl_skin* s, const double* d_positions, void* stream, double* max_disp,
+                  int* needs_rebuild);
+int64_t nl_skin_num_edges(const nl_skin* s, size_t list);
+int nl_skin_device(const nl_skin* s, size_t list, const int64_t** edge_index,
+                   const int32_t** shifts);
+/* updates since creation, builds (the first included), last measured
+ * largest displacement */
+int nl_skin_stats(const nl_skin* s, int64_t* updates, int64_t* rebuilds, double* last_disp);
+void nl_skin_free(nl_skin* s);
+
 /* Host-side cell helpers with the reference's arithmetic (no GPU needed):
  * H^-1 by the cofactor formula of Cell::new (cell.rs:18-21; returns nonzero
  * with "Cell matrix is not invertible" when det == 0) and Cell::wrap

@@ -11,6 +11,7 @@
 #include "../../include/nl_engine_ext.h"
 #include "nl_engine.h"
 #include "nl_server.h"
+#include "nl_skin.h"
 
 using nl::Engine;
 using nl::Request;
@@ -28,6 +29,11 @@ struct nl_plan {
 
 struct nl_server {
   nl::Server* s;
+};
+
+struct nl_skin {
+  nl::Skin k;
+  nl::Engine* eng;
 };
 
 namespace {
@@ -537,6 +543,115 @@ size_t nl_tune_report(char* buf, size_t len) {
   }
   return r.size();
 }
+
+// ---- skin lists for molecular dynamics --------------------------------------
+int nl_skin_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                   const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                   const double* cutoffs, const double* skins, size_t n_cutoffs, int disjoint,
+                   int multi, unsigned options, void* stream, nl_skin** out) {
+  if (!out) {
+    nl::set_error("output pointer is NULL");
+    return 1;
+  }
+  *out = nullptr;
+  if (check_cutoffs(cutoffs, n_cutoffs, "All cutoffs must be positive and finite")) return 1;
+  if (n == 0 || n_cutoffs == 0) {
+    nl::set_error("skin lists need at least one atom and one cutoff");
+    return 1;
+  }
+  if (!skins) {
+    nl::set_error("skins is NULL");
+    return 1;
+  }
+  if (options & ~(unsigned)(NL_DEVICE_POSITIONS | NL_LOCAL_INDEX | NL_FULL)) {
+    nl::set_error("skin lists take NL_LOCAL_INDEX and NL_FULL only");
+    return 1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    nl::set_error("no HIP device available (libnlmi355 runs on the GPU only)");
+    return 1;
+  }
+  Request req;
+  if (sizes_request(d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, req)) return 1;
+  req.pos_device = true;
+  req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
+  req.multi = multi != 0;
+  req.disjoint = disjoint != 0;
+  req.global_index = (options & NL_LOCAL_INDEX) == 0;
+  req.full = (options & NL_FULL) != 0;
+  nl_skin* k = new nl_skin();
+  k->eng = nl::engine_for_current_device();
+  int rc;
+  {
+    std::lock_guard<std::mutex> g(k->eng->mutex());
+    rc = k->k.create(k->eng, req, std::vector<double>(skins, skins + n_cutoffs),
+                     (hipStream_t)stream);
+  }
+  if (rc != 0) {
+    delete k;
+    return rc;
+  }
+  *out = k;
+  return 0;
+}
+
+int nl_skin_update(nl_skin* s, const double* d_positions, const double* cells, void* stream,
+                   int* rebuilt) {
+  if (!s || !d_positions) {
+    nl::set_error("invalid skin lists or positions");
+    return 1;
+  }
+  bool r = false;
+  int rc;
+  {
+    std::lock_guard<std::mutex> g(s->eng->mutex());
+    rc = s->k.update(d_positions, cells, (hipStream_t)stream, &r);
+  }
+  if (rebuilt) *rebuilt = r ? 1 : 0;
+  return rc;
+}
+
+int nl_skin_check(nl_skin* s, const double* d_positions, void* stream, double* max_disp,
+                  int* needs_rebuild) {
+  if (!s || !d_positions) {
+    nl::set_error("invalid skin lists or positions");
+    return 1;
+  }
+  bool needs = false;
+  const int rc = s->k.check(d_positions, (hipStream_t)stream, max_disp, &needs);
+  if (needs_rebuild) *needs_rebuild = needs ? 1 : 0;
+  return rc;
+}
+
+int64_t nl_skin_num_edges(const nl_skin* s, size_t list) {
+  if (!s || list >= (size_t)s->k.lists()) return -1;
+  return s->k.len((int)list);
+}
+
+int nl_skin_device(const nl_skin* s, size_t list, const int64_t** edge_index,
+                   const int32_t** shifts) {
+  if (!s || list >= (size_t)s->k.lists()) {
+    nl::set_error("invalid skin lists or list index");
+    return 1;
+  }
+  if (edge_index) *edge_index = s->k.edges((int)list);
+  if (shifts) *shifts = s->k.shifts((int)list);
+  return 0;
+}
+
+int nl_skin_stats(const nl_skin* s, int64_t* updates, int64_t* rebuilds, double* last_disp) {
+  if (!s) {
+    nl::set_error("invalid skin lists");
+    return 1;
+  }
+  if (updates) *updates = s->k.updates;
+  if (rebuilds) *rebuilds = s->k.rebuilds;
+  if (last_disp) *last_disp = s->k.last_disp;
+  return 0;
+}
+
+void nl_skin_free(nl_skin* s) { delete s; }
 
 int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
                    const double* cells, const uint8_t* has_cell, const uint8_t* pbc,

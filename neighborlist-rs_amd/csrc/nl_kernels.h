@@ -64,6 +64,9 @@ hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const S
 hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
                        int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st);
+// *out = bits of max_a |pos_a - ref_a|^2 (cleared first, in the same stream)
+hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
+                            unsigned long long* out, hipStream_t st);
 hipError_t launch_sys_ranges(const int64_t* offsets, const int64_t* item_base, int n_sys, int k,
                              int64_t n, int64_t* ranges, hipStream_t st);
 

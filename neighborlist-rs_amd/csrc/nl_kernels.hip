@@ -1416,6 +1416,39 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
   }
 }
 
+// Largest squared displacement |pos - ref|^2 over n atoms, into *out as the
+// bit pattern of a non-negative double (ordered like the value, so an
+// unsigned atomic max finds it).  *out must be zero before the launch.
+__global__ __launch_bounds__(256) void k_max_disp2(const double* __restrict__ pos,
+                                                   const double* __restrict__ ref, int64_t n,
+                                                   unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_w[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  unsigned long long best = 0ull;
+  for (int64_t a = (int64_t)blockIdx.x * blockDim.x + tid; a < n;
+       a += (int64_t)gridDim.x * blockDim.x) {
+    const double dx = pos[3 * a] - ref[3 * a], dy = pos[3 * a + 1] - ref[3 * a + 1],
+                 dz = pos[3 * a + 2] - ref[3 * a + 2];
+    const double d2 = (dx * dx + dy * dy) + dz * dz;
+    // NaN positions count as an unbounded displacement
+    const unsigned long long b = d2 == d2 ? (unsigned long long)__double_as_longlong(d2)
+                                          : 0x7ff0000000000000ull;
+    best = b > best ? b : best;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long t = __shfl_xor(best, o, 64);
+    best = t > best ? t : best;
+  }
+  if (lane == 0) s_w[w] = best;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long m = s_w[0];
+    for (int i = 1; i < 4; ++i) m = s_w[i] > m ? s_w[i] : m;
+    if (m) atomicMax(out, m);
+  }
+}
+
 // Per-system edge ranges: ranges[(s*k + q)*2 + {0,1}].
 __global__ void k_sys_ranges(const int64_t* __restrict__ off,
                              const int64_t* __restrict__ item_base, int n_sys, int k, int64_t n,
@@ -1582,6 +1615,15 @@ hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t se
   hipLaunchKernelGGL(k_scan_sums_excl, dim3(1), dim3(1024), 0, st, status, tiles);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len, seg,
                      nseg, bounds, (const uint64_t*)status);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
+                            unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t*)out, (int64_t)2, 0u);
+  if (n <= 0) return hipGetLastError();
+  const int64_t blocks = std::min<int64_t>(grid_for(n, 256), 2048);
+  hipLaunchKernelGGL(k_max_disp2, dim3((unsigned)blocks), dim3(256), 0, st, pos, ref, n, out);
   return hipGetLastError();
 }
 

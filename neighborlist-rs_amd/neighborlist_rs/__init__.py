@@ -399,3 +399,4 @@ def set_stack_threshold(val):
 from .serving import Server  # noqa: E402  (continuous-batching server)
 from .tensors import (build_neighborlists_torch, build_neighborlists_multi_torch,  # noqa: E402
                       build_neighborlists_batch_torch, build_neighborlists_batch_multi_torch)
+from .md import DynamicNeighborManager  # noqa: E402  (skin lists for MD)

@@ -93,6 +93,16 @@ _SIGS = {
     "nl_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "nl_server_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p, i64p]),
     "nl_server_free": (None, [C.c_void_p]),
+    "nl_skin_create": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp, dp,
+                                 C.c_size_t, C.c_int, C.c_int, C.c_uint, C.c_void_p,
+                                 C.POINTER(C.c_void_p)]),
+    "nl_skin_update": (C.c_int, [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.POINTER(C.c_int)]),
+    "nl_skin_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, dp, C.POINTER(C.c_int)]),
+    "nl_skin_num_edges": (C.c_int64, [C.c_void_p, C.c_size_t]),
+    "nl_skin_device": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p)]),
+    "nl_skin_stats": (C.c_int, [C.c_void_p, i64p, i64p, dp]),
+    "nl_skin_free": (None, [C.c_void_p]),
     "nl_cell_inverse": (C.c_int, [dp, dp]),
     "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
     "nl_device_malloc": (C.c_void_p, [C.c_size_t]),
