@@ -798,6 +798,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   const float iwz = 1.0f / (float)wz, iwy = 1.0f / (float)wy;
   const float inx = 1.0f / (float)nbx, iny = 1.0f / (float)nby, inz = 1.0f / (float)nbz;
   // shift by whole periods so that the bin index is non-negative
+  // an atom can meet its own image only when the stencil spans a period
+  const bool self_possible = nsx >= nbx || nsy >= nby || nsz >= nbz;
   const int kx = (nsx + nbx - 1) / nbx + 1, ky = (nsy + nby - 1) / nby + 1,
             kz = (nsz + nbz - 1) / nbz + 1;
   // stencil pruning: perpendicular bin widths, cell shape, bound on d^2
@@ -933,7 +935,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
           const int csg = sg, cjs = j;
           const bool cvalid = valid;
           const float4 c32 = n32;
-          const Pos64 c = n64;
+          Pos64 c = n64;
           const int4 cj = nij;
           if (c0 + 64 < total) {
             locate(c0 + 64);
@@ -950,14 +952,22 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             py = c32.y + L.off32[1][csg];
             pz = c32.z + L.off32[2][csg];
           }
+          // lanes past the last candidate: a NaN coordinate fails every test
+          if (!cvalid) {
+            c.x = NAN;
+            px = NAN;
+          }
           const int oj = F32 ? __float_as_int(c32.w) : __double2loint(c.w);
           // home atom i is tested against this lane's candidate iff i < lim
           // (in the home bin without image only the later slots)
           const int lim = cvalid ? (home_seg ? cjs - hb : 64) : 0;
           // the home-atom loop, compiled once per disjoint mode (a uniform
           // branch inside it costs register moves and selects on every step)
-          auto home_loop = [&](auto disj_tag) {
+          // and once for chunks where every lane tests every home atom: no
+          // home-bin lanes (slot limit) and no own images possible
+          auto home_loop = [&](auto disj_tag, auto simple_tag) {
             constexpr bool DISJ = decltype(disj_tag)::value;
+            constexpr bool SIMPLE = decltype(simple_tag)::value;
 #pragma unroll 1
             for (int i = 0; i < hn; ++i) {
               // broadcast LDS reads of the home atom (uniform address)
@@ -968,14 +978,15 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
               else
                 h64 = L.home64[i];
               const int oi = F32 ? __float_as_int(h32.w) : __double2loint(h64.w);
-              const bool cand = i < lim;
+              const bool cand = SIMPLE || i < lim;
               if (!FILL && !DISJ) {
                 // Count pass, cumulative lists: lane counters, no ballots.  A
                 // self image (same atom, other image) counts twice in a half
                 // list (both S and -S are emitted); full lists count every hit
                 // once and double the item's count at the end.  Lists past
                 // q_cnt have -inf thresholds.
-                const uint32_t w1 = cand ? 1u + (uint32_t)(!FULL && oj == oi) : 0u;
+                const uint32_t w1 =
+                    SIMPLE ? 1u : (cand ? 1u + (uint32_t)(!FULL && oj == oi) : 0u);
                 if (F32) {
                   const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
                   const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -1035,7 +1046,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                 if (cand) mem = members64<DISJ, NQ>(d2, C);
               }
               if (FILL && __ballot(mem != 0) == 0) continue;
-              const bool self_img = (mem != 0) && (oj == oi);
+              const bool self_img = !SIMPLE && (mem != 0) && (oj == oi);
               int sx = 0, sy = 0, sz = 0;
               int64_t gi = 0, gj = 0;
               if (FILL) {
@@ -1076,15 +1087,23 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
               // the self image's -S copy (full lists get it from the mirror;
               // their count pass counts each half-stencil hit once, doubled
               // when the item ends)
-              if (!FULL && (!FILL || __ballot(self_img)))
+              if (!FULL && !SIMPLE && (!FILL || __ballot(self_img)))
                 emit_step<FILL, NQ, false, DISP>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz,
                                                  d2, -ex, -ey, -ez, cur);
             }
           };
-          if (C.disjoint)
-            home_loop(std::true_type{});
-          else
-            home_loop(std::false_type{});
+          const bool simple = !self_possible && __ballot(home_seg) == 0;
+          if (C.disjoint) {
+            if (simple)
+              home_loop(std::true_type{}, std::true_type{});
+            else
+              home_loop(std::true_type{}, std::false_type{});
+          } else {
+            if (simple)
+              home_loop(std::false_type{}, std::true_type{});
+            else
+              home_loop(std::false_type{}, std::false_type{});
+          }
         }
       }
       __builtin_amdgcn_wave_barrier();
