@@ -21,6 +21,9 @@ constexpr int kRadixTile = 2048;    // keys per radix / scan tile (256 threads x
 // scans of up to this many tiles use one decoupled look-back launch; longer
 // ones reduce-then-scan (three launches, no block waits on another)
 constexpr int kScanChainTiles = 32;
+// segmented sort of small systems: atoms per system, digit width
+constexpr int kSegMax = 4096;
+constexpr int kSegBits = 7;
 
 // Search mode chosen per system, exactly as single.rs:60-89 / batch.rs:100-124.
 enum Mode : int32_t {

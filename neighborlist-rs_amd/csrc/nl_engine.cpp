@@ -155,6 +155,15 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
     const int kb = bits_for(bin_total - 1);
     passes = (kb + kRadixMaxBits - 1) / kRadixMaxBits;
     digit_bits = (kb + passes - 1) / passes;
+    // batches of small systems: one block sorts each system's local keys
+    int64_t max_n = 0, max_cap = 1;
+    for (int s = 0; s < S; ++s) {
+      max_n = std::max<int64_t>(max_n, desc[s].n_atoms);
+      max_cap = std::max<int64_t>(max_cap, desc[s].bin_cap);
+    }
+    seg_sort = max_n <= kSegMax;
+    const int lb = bits_for(max_cap - 1);
+    seg_passes = (lb + kSegBits - 1) / kSegBits;
   }
   rtiles = radix_tiles(n);
   // work items: per system its atoms (brute-force rows), then
@@ -263,8 +272,9 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.sys = b.sys;
   A.sys_start = b.start;
   A.item_base = b.items;
-  // the sorted keys end in A after an even number of radix passes
-  A.keys = (P.passes % 2 == 0) ? b.keysA : b.keysB;
+  // the sorted keys end in A after an even number of radix passes (in B
+  // after the segmented sort)
+  A.keys = P.seg_sort ? b.keysB : ((P.passes % 2 == 0) ? b.keysA : b.keysB);
   A.cell_start = b.cell_start;
   A.spos64 = b.spos64;
   A.spos32 = b.spos32;
@@ -298,7 +308,12 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   NL_TRY(launch_plan(b.pos, b.sys, b.band, b.rc2, prm, st));
   NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, st));
   uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
-  for (int ps = 0; ps < passes; ++ps) {
+  if (seg_sort) {
+    NL_TRY(launch_sort_systems(kin, vin, kout, vout, b.sys, b.start, S, seg_passes, st));
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  for (int ps = 0; ps < (seg_sort ? 0 : passes); ++ps) {
     NL_TRY(launch_radix_pass(kin, vin, kout, vout, n, digit_bits * ps, digit_bits, b.rcnt,
                              b.roff, b.rbounds, rstatus + rstiles * ps, tickets + ps, st));
     std::swap(kin, kout);

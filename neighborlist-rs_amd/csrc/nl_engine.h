@@ -115,6 +115,10 @@ class Pipeline {
   int64_t n = 0, n_items = 0, bin_total = 0, scan_len = 0, rtiles = 0, stiles = 0,
           rstiles = 0, z_words = 0;
   int S = 0, K = 0, passes = 0, digit_bits = 8;
+  // every system fits one block: one segmented sort launch (seg_passes
+  // digit passes in LDS) instead of the global radix passes
+  bool seg_sort = false;
+  int seg_passes = 0;
   Tuning tune;
   Params prm{};
   bool pos_device = false, global_index = true, disjoint = false, per_sys = false, full = false;

@@ -67,6 +67,10 @@ hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t se
 // *out = bits of max_a |pos_a - ref_a|^2 (cleared first, in the same stream)
 hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
                             unsigned long long* out, hipStream_t st);
+// stable sort of each system's keys (<= kSegMax atoms each) in one launch
+hipError_t launch_sort_systems(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                               uint32_t* vout, const SysDesc* sys, const int64_t* sys_start,
+                               int n_sys, int passes, hipStream_t st);
 hipError_t launch_sys_ranges(const int64_t* offsets, const int64_t* item_base, int n_sys, int k,
                              int64_t n, int64_t* ranges, hipStream_t st);
 

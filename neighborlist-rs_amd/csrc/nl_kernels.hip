@@ -483,6 +483,111 @@ __global__ __launch_bounds__(256) void k_radix_small(const uint32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Segmented sort for batches of small systems: one block per system (at most
+// kSegMax atoms), local bin keys (key - bin_base) sorted in LDS by stable LSD
+// passes of kSegBits-bit digits, the result written back with the global
+// keys.  Systems are contiguous and their bin ranges ascend, so the output is
+// the globally sorted order the global radix sort would give, in one launch
+// instead of three per pass.
+__global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict__ kin,
+                                                      const uint32_t* __restrict__ vin,
+                                                      uint32_t* __restrict__ kout,
+                                                      uint32_t* __restrict__ vout,
+                                                      const SysDesc* __restrict__ sys,
+                                                      const int64_t* __restrict__ sys_start,
+                                                      int passes) {
+  constexpr int ND = 1 << kSegBits;
+  __shared__ uint32_t s_k[2][kSegMax];
+  __shared__ uint32_t s_v[2][kSegMax];
+  __shared__ uint32_t s_cnt[4][ND];  // per-wave digit counters of the current tile
+  __shared__ uint32_t s_run[ND];     // digit offsets so far
+  __shared__ uint32_t s_tmp[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int s = blockIdx.x;
+  const int64_t a0 = sys_start[s];
+  const int n = (int)(sys_start[s + 1] - a0);
+  const uint32_t base = (uint32_t)sys[s].bin_base;
+  for (int i = tid; i < n; i += 256) {
+    s_k[0][i] = kin[a0 + i] - base;
+    s_v[0][i] = vin[a0 + i];
+  }
+  int cur = 0;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kSegBits;
+    const bool last = p == passes - 1;
+    for (int d = tid; d < ND; d += 256) {
+      s_run[d] = 0u;
+      s_cnt[0][d] = s_cnt[1][d] = s_cnt[2][d] = s_cnt[3][d] = 0u;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) atomicAdd(&s_run[(s_k[cur][i] >> shift) & (ND - 1)], 1u);
+    __syncthreads();
+    {
+      const uint32_t c = tid < ND ? s_run[tid] : 0u;
+      const uint32_t x = block_excl_scan_256(c, s_tmp, nullptr);
+      __syncthreads();
+      if (tid < ND) s_run[tid] = x;
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < n; t0 += 2048) {
+      uint32_t kk[8], vv[8], lp[8];
+      // stable ranks: wave w takes elements t0 + 512 w + 64 r + lane in order
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int e = t0 + 512 * w + 64 * r + lane;
+        const bool valid = e < n;
+        const uint32_t key = valid ? s_k[cur][e] : 0u;
+        const uint32_t dig = (key >> shift) & (ND - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kSegBits; ++b) {
+          const bool bit = (dig >> b) & 1u;
+          const uint64_t bb = __ballot(bit);
+          peers &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = lanes_below(peers);
+        const uint32_t before = valid ? s_cnt[w][dig] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) s_cnt[w][dig] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        lp[r] = before + rank;
+        kk[r] = key;
+        vv[r] = valid ? s_v[cur][e] : 0u;
+      }
+      __syncthreads();
+      for (int d = tid; d < ND; d += 256) {
+        const uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
+        const uint32_t run = s_run[d];
+        s_cnt[0][d] = run;
+        s_cnt[1][d] = run + c0;
+        s_cnt[2][d] = run + c0 + c1;
+        s_cnt[3][d] = run + c0 + c1 + c2;
+        s_run[d] = run + c0 + c1 + c2 + c3;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int e = t0 + 512 * w + 64 * r + lane;
+        if (e < n) {
+          const uint32_t dst = s_cnt[w][(kk[r] >> shift) & (ND - 1)] + lp[r];
+          if (last) {
+            kout[a0 + dst] = kk[r] + base;
+            vout[a0 + dst] = vv[r];
+          } else {
+            s_k[cur ^ 1][dst] = kk[r];
+            s_v[cur ^ 1][dst] = vv[r];
+          }
+        }
+      }
+      __syncthreads();
+      for (int d = tid; d < ND; d += 256) s_cnt[0][d] = s_cnt[1][d] = s_cnt[2][d] = s_cnt[3][d] = 0u;
+      __syncthreads();
+    }
+    cur ^= 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Finalize: thread t writes the offset of bin t (lower bound of t in the
 // sorted keys, for t <= total_bins) and, for t < n, the sorted slot t: the
 // wrapped position and wrap shift of its atom, as CellList::build computes
@@ -1643,6 +1748,15 @@ hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
   if (n <= 0) return hipGetLastError();
   const int64_t blocks = std::min<int64_t>(grid_for(n, 256), 2048);
   hipLaunchKernelGGL(k_max_disp2, dim3((unsigned)blocks), dim3(256), 0, st, pos, ref, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_systems(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                               uint32_t* vout, const SysDesc* sys, const int64_t* sys_start,
+                               int n_sys, int passes, hipStream_t st) {
+  if (n_sys <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_systems, dim3((unsigned)n_sys), dim3(256), 0, st, kin, vin, kout, vout,
+                     sys, sys_start, passes);
   return hipGetLastError();
 }
 
