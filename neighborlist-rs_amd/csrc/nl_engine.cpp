@@ -309,7 +309,8 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, st));
   uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
   if (seg_sort) {
-    NL_TRY(launch_sort_systems(kin, vin, kout, vout, b.sys, b.start, S, seg_passes, st));
+    NL_TRY(launch_sort_systems(kin, vin, kout, vout, b.sys, b.start, S, seg_passes,
+                               b.cell_start, st));
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
@@ -319,8 +320,9 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
+  // (the segmented sort has written the bin offsets already)
   NL_TRY(launch_finalize(kin, vin, b.pos, b.sys, b.start, S, n, bin_total, b.cell_start,
-                         b.spos64, b.spos32, b.sidx, st));
+                         b.spos64, b.spos32, b.sidx, !seg_sort, st));
   // work list of the items that have work
   NL_TRY(launch_items(kin, b.cell_start, b.sys, b.items, S, n_items, tune.group, tune.parts,
                       b.list, tickets + 6, st));

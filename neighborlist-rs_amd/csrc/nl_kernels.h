@@ -57,7 +57,7 @@ hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t*
 hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const double* pos,
                            const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,
                            int64_t total_bins, int32_t* cell_start, Pos64* spos64,
-                           float4* spos32, int4* sidx, hipStream_t st);
+                           float4* spos32, int4* sidx, bool with_cells, hipStream_t st);
 hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const SysDesc* sys,
                         const int64_t* item_base, int n_sys, int64_t n_items, int group,
                         int parts, int64_t* list, uint32_t* count, hipStream_t st);
@@ -70,7 +70,7 @@ hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
 // stable sort of each system's keys (<= kSegMax atoms each) in one launch
 hipError_t launch_sort_systems(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                uint32_t* vout, const SysDesc* sys, const int64_t* sys_start,
-                               int n_sys, int passes, hipStream_t st);
+                               int n_sys, int passes, int32_t* cell_start, hipStream_t st);
 hipError_t launch_sys_ranges(const int64_t* offsets, const int64_t* item_base, int n_sys, int k,
                              int64_t n, int64_t* ranges, hipStream_t st);
 

@@ -488,14 +488,17 @@ __global__ __launch_bounds__(256) void k_radix_small(const uint32_t* __restrict_
 // passes of kSegBits-bit digits, the result written back with the global
 // keys.  Systems are contiguous and their bin ranges ascend, so the output is
 // the globally sorted order the global radix sort would give, in one launch
-// instead of three per pass.
+// instead of three per pass.  The block also writes the bin offsets of its
+// system (cell_start[bin_base + b] for b <= bin_cap) from the sorted keys in
+// LDS, which k_finalize then skips.
 __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict__ kin,
                                                       const uint32_t* __restrict__ vin,
                                                       uint32_t* __restrict__ kout,
                                                       uint32_t* __restrict__ vout,
                                                       const SysDesc* __restrict__ sys,
                                                       const int64_t* __restrict__ sys_start,
-                                                      int passes) {
+                                                      int passes,
+                                                      int32_t* __restrict__ cell_start) {
   constexpr int ND = 1 << kSegBits;
   __shared__ uint32_t s_k[2][kSegMax];
   __shared__ uint32_t s_v[2][kSegMax];
@@ -514,7 +517,6 @@ __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict
   int cur = 0;
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kSegBits;
-    const bool last = p == passes - 1;
     for (int d = tid; d < ND; d += 256) {
       s_run[d] = 0u;
       s_cnt[0][d] = s_cnt[1][d] = s_cnt[2][d] = s_cnt[3][d] = 0u;
@@ -570,13 +572,8 @@ __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict
         const int e = t0 + 512 * w + 64 * r + lane;
         if (e < n) {
           const uint32_t dst = s_cnt[w][(kk[r] >> shift) & (ND - 1)] + lp[r];
-          if (last) {
-            kout[a0 + dst] = kk[r] + base;
-            vout[a0 + dst] = vv[r];
-          } else {
-            s_k[cur ^ 1][dst] = kk[r];
-            s_v[cur ^ 1][dst] = vv[r];
-          }
+          s_k[cur ^ 1][dst] = kk[r];
+          s_v[cur ^ 1][dst] = vv[r];
         }
       }
       __syncthreads();
@@ -584,6 +581,23 @@ __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict
       __syncthreads();
     }
     cur ^= 1;
+  }
+  for (int i = tid; i < n; i += 256) {
+    kout[a0 + i] = s_k[cur][i] + base;
+    vout[a0 + i] = s_v[cur][i];
+  }
+  // bin offsets: first slot whose local key is >= b
+  const int cap = sys[s].bin_cap;
+  for (int b = tid; b <= cap; b += 256) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)s_k[cur][mid] < b)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    cell_start[(int64_t)base + b] = (int32_t)(a0 + lo);
   }
 }
 
@@ -597,9 +611,9 @@ __global__ __launch_bounds__(256) void k_finalize(
     const double* __restrict__ pos, const SysDesc* __restrict__ sys,
     const int64_t* __restrict__ sys_start, int n_sys, int64_t n, int64_t total_bins,
     int32_t* __restrict__ cell_start, Pos64* __restrict__ spos64, float4* __restrict__ spos32,
-    int4* __restrict__ sidx) {
+    int4* __restrict__ sidx, int with_cells) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t <= total_bins) {
+  if (with_cells && t <= total_bins) {
     int64_t lo = 0, hi = n;  // first slot with key >= t
     while (lo < hi) {
       const int64_t mid = (lo + hi) >> 1;
@@ -1654,11 +1668,12 @@ hipError_t launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t*
 hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const double* pos,
                            const SysDesc* sys, const int64_t* sys_start, int n_sys, int64_t n,
                            int64_t total_bins, int32_t* cell_start, Pos64* spos64,
-                           float4* spos32, int4* sidx, hipStream_t st) {
+                           float4* spos32, int4* sidx, bool with_cells, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const int64_t threads = n > total_bins + 1 ? n : total_bins + 1;
+  const int64_t threads = (with_cells && total_bins + 1 > n) ? total_bins + 1 : n;
   hipLaunchKernelGGL(k_finalize, dim3(grid_for(threads, 256)), dim3(256), 0, st, keys, vals, pos, sys,
-                     sys_start, n_sys, n, total_bins, cell_start, spos64, spos32, sidx);
+                     sys_start, n_sys, n, total_bins, cell_start, spos64, spos32, sidx,
+                     with_cells ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1753,10 +1768,10 @@ hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
 
 hipError_t launch_sort_systems(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                uint32_t* vout, const SysDesc* sys, const int64_t* sys_start,
-                               int n_sys, int passes, hipStream_t st) {
+                               int n_sys, int passes, int32_t* cell_start, hipStream_t st) {
   if (n_sys <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sort_systems, dim3((unsigned)n_sys), dim3(256), 0, st, kin, vin, kout, vout,
-                     sys, sys_start, passes);
+                     sys, sys_start, passes, cell_start);
   return hipGetLastError();
 }
 
