@@ -280,3 +280,27 @@ def test_large_single_system_vs_oracle(nr, oracle):
     want = oracle.search_single(p, H, [1, 1, 1], 5.0)
     got = _gpu_single(nr, p, H, [True] * 3, 5.0)
     assert_same_edges(got, want, "60k triclinic rc=5")
+
+
+@pytest.mark.parametrize("sizes", [(4096, 1, 4000, 37), (4096, 4097, 1, 2, 700)])
+def test_segmented_sort_boundary_vs_oracle(nr, oracle, sizes):
+    """Batches whose systems all have at most 4096 atoms are sorted one
+    system per block (segmented sort); one larger system sends the whole
+    batch through the global radix sort.  Both sides of that boundary, with
+    periodic, isolated and single-atom systems."""
+    from neighborlist_rs import workloads as W
+    rng = np.random.default_rng(sum(sizes))
+    ps, cells = [], []
+    for s, n in enumerate(sizes):
+        p, H = W.random_system(rng, n, s % 3)
+        ps.append(p)
+        cells.append(H)
+    p = np.ascontiguousarray(np.concatenate(ps))
+    batch, rows = W.batch_arrays(np.array(sizes), cells)
+    _print_case("segmented sort boundary", sizes)
+    ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+    cut = [3.5, 6.0]
+    want = oracle.search_batch_multi(p, batch, ocells, cut, False)
+    got = nr.build_neighborlists_batch_multi(p, batch, rows, cut)
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"{sizes} list {k}")
