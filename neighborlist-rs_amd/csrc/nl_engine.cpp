@@ -701,11 +701,12 @@ Tuning Engine::tuning_for(const Request& req) {
     x.use_fp32 = 1 - x.use_fp32;
     cand.insert(cand.begin() + 1, {time_one(x, 3), x});
   }
-  // ... and the leading three re-timed in turn, best of five each, twice
+  // ... and the leading three re-timed in turn, best of seven each, three
+  // rounds (they are often within a few per cent of each other)
   const int top = std::min<int>(3, (int)cand.size());
   std::vector<double> tt(top, 1e30);
-  for (int round = 0; round < 2; ++round)
-    for (int i = 0; i < top; ++i) tt[i] = std::min(tt[i], time_one(cand[i].second, 5));
+  for (int round = 0; round < 3; ++round)
+    for (int i = 0; i < top; ++i) tt[i] = std::min(tt[i], time_one(cand[i].second, 7));
   int bi = 0;
   for (int i = 1; i < top; ++i)
     if (tt[i] < tt[bi]) bi = i;
