@@ -586,18 +586,23 @@ __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict
     kout[a0 + i] = s_k[cur][i] + base;
     vout[a0 + i] = s_v[cur][i];
   }
-  // bin offsets: first slot whose local key is >= b
+  // bin offsets, cell_start[base + b] = first slot whose local key is >= b:
+  // slot i starts the bins (key[i-1], key[i]]; the bins up to the first key
+  // start at slot 0 and those past the last key at n
+  // (an empty system: every bin starts at slot 0 = n)
+  __syncthreads();  // with no radix pass the loaded keys are still per-thread
   const int cap = sys[s].bin_cap;
+  const int kfirst = n > 0 ? (int)s_k[cur][0] : cap;
+  const int klast = n > 0 ? (int)s_k[cur][n - 1] : cap;
   for (int b = tid; b <= cap; b += 256) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if ((int)s_k[cur][mid] < b)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    cell_start[(int64_t)base + b] = (int32_t)(a0 + lo);
+    if (b <= kfirst)
+      cell_start[(int64_t)base + b] = (int32_t)a0;
+    else if (b > klast)
+      cell_start[(int64_t)base + b] = (int32_t)(a0 + n);
+  }
+  for (int i = tid + 1; i < n; i += 256) {
+    const int k = (int)s_k[cur][i], prev = (int)s_k[cur][i - 1];
+    for (int b = prev + 1; b <= k; ++b) cell_start[(int64_t)base + b] = (int32_t)(a0 + i);
   }
 }
 
