@@ -303,6 +303,76 @@ def serve_trace(L, n_req, slots, load):
     }
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def ethanol_calibration(lib, L, O, threads):
+    """The reference's own published case, ethanol_10000 (tests/data of the
+    reference, PBC cube) at rc 6: the oracle at `threads` threads and at one
+    thread beside the reference's logged times, and the GPU on the same input
+    (device-resident positions and lists; and numpy in, numpy out through the
+    Python API, host copies included)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from xyzio import read_xyz
+    import neighborlist_rs as nr
+    pos, rows, pbc = read_xyz("ethanol_10000.xyz")
+    H = np.ascontiguousarray(rows.T)  # lattice vectors as columns
+    n = pos.shape[0]
+    d = DevArray(lib, pos)
+    sz = np.array([n], np.int64)
+    H9 = np.ascontiguousarray(H.reshape(1, 9))
+    has = np.ones(1, np.uint8)
+    pb = np.ascontiguousarray(np.array(pbc, np.uint8).reshape(1, 3))
+    cut = np.array([6.0])
+    for _ in range(3):
+        run_systems(lib, L, d.ptr, n, sz, H9, has, pb, cut, False)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        e = run_systems(lib, L, d.ptr, n, sz, H9, has, pb, cut, False)
+    gpu_ms = 1e3 * (time.perf_counter() - t0) / 20
+    d.free()
+    cell = nr.PyCell(H.tolist(), [bool(x) for x in pbc])
+    for _ in range(3):
+        nr.build_neighborlists(cell, pos, 6.0)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        r = nr.build_neighborlists(cell, pos, 6.0)
+    api_ms = 1e3 * (time.perf_counter() - t0) / 10
+    assert r["edge_index"].shape[1] == e
+    O.set_threads(threads)
+    O.search_single(pos, H, pbc, 6.0)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        ce = len(O.search_single(pos, H, pbc, 6.0)[0])
+    cpu_ms = 1e3 * (time.perf_counter() - t0) / 10
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        O.search_single(pos, H, pbc, 6.0)
+    cpu1_ms = 1e3 * (time.perf_counter() - t0) / 3
+    O.set_threads(threads)
+    assert ce == e, (ce, e)
+    return {"fixture": f"ethanol_10000.xyz ({n} atoms, periodic cube), rc 6", "edges": int(e),
+            "oracle_ms": round(cpu_ms, 3), "oracle_threads": threads,
+            "oracle_1_thread_ms": round(cpu1_ms, 3), "cpu_model": cpu_model(),
+            "reference_published_ms": {
+                "grace_20_threads": 7.32, "grace_1_thread": 27.98, "epyc7r13_64_threads": 11.46,
+                "source": "benchmarks/benchmark_grace.txt:39-40, "
+                          "benchmarks/benchmark_EPYC7R13.txt:40 (Python API, numpy out)"},
+            "gpu_device_ms": round(gpu_ms, 4),
+            "gpu_python_api_ms": round(api_ms, 4),
+            "gpu_python_api_note": "build_neighborlists, numpy in and out (H2D and D2H "
+                                   "included), as the published times"}
+
+
 def precision_report(L, pos, H, rc):
     """FP64 |r_ij| emitted with the edges (cell-list path) against a float64
     recomputation from the raw positions, |p_j + H S - p_i|, for the FP32
@@ -568,6 +638,7 @@ def main():
 
     cpu = None
     c3_cpu_ms = None
+    calibration = None
     if not args.no_cpu:
         from oracle import oracle as O
         threads = O.max_threads()
@@ -593,7 +664,10 @@ def main():
         for _ in range(10):
             O.search_single(p3, H3, [1, 1, 1], 6.0)
         c3_cpu_ms = 1e3 * (time.perf_counter() - t0) / 10
+        cpu["cpu_model"] = cpu_model()
         log(f"cpu: {cpu['value']:.3e} edges/s on {threads} threads; c3 rc6 {c3_cpu_ms:.2f} ms")
+        calibration = ethanol_calibration(lib, L, O, threads)
+        log(f"ethanol_10000 rc 6: {calibration}")
 
     out = {
         "metric": "edges/s and systems/s (batched/served vs plain GPU); p50/p99 build latency",
@@ -633,6 +707,7 @@ def main():
         "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ethanol_10000_rc6_calibration": calibration,
     }
     dpos.free()
     d3.free()

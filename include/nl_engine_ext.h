@@ -130,6 +130,24 @@ int nl_plan_copy(const nl_plan* p, size_t list, int64_t* edge_i, int64_t* edge_j
                  int32_t* shifts);
 int nl_plan_stats(const nl_plan* p, int64_t* replays, int64_t* recaptures, int64_t* capacity);
 void nl_plan_free(nl_plan* p);
+/* The general form of nl_plan_create: options are the NL_* flags of
+ * nl_search_ex (NL_LOCAL_INDEX, NL_RANGES, NL_DIST, NL_FULL, NL_DISP;
+ * positions are device-resident in any case), captured into the graph with
+ * the rest, so a replay also emits full lists, distances or displacements
+ * and downloads the per-system ranges. */
+int nl_plan_create_ex(const double* d_positions, size_t n, const int64_t* sys_sizes,
+                      size_t n_sys, const double* cells, const uint8_t* has_cell,
+                      const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs, int disjoint,
+                      int multi, unsigned options, int64_t capacity, nl_plan** out);
+/* Device address of one array of list `list` after the last run: what = 0
+ * edge_index (2, M) int64, 1 shift (M, 3) int32, 2 distance (M,) float64
+ * (NL_DIST), 3 displacement (M, 3) float64 (NL_DISP).  Valid until the next
+ * nl_plan_run (which moves the arrays when the lists outgrow the capacity)
+ * or nl_plan_free. */
+int nl_plan_array(const nl_plan* p, size_t list, int what, const void** ptr);
+/* Per-system ranges and modes of the last run, as nl_result_system_ranges
+ * (NL_RANGES). */
+int nl_plan_system_ranges(const nl_plan* p, int64_t* ranges, int32_t* modes);
 
 /* ---- serving: continuous batching -----------------------------------------
  * No counterpart in the reference (its API is synchronous, one call per

@@ -1,6 +1,7 @@
 // nl_capi.cpp -- extern "C" layer of libnlmi355 (include/nl_api.h,
 // include/nl_engine_ext.h).  Validation and segmentation follow the
 // reference's dispatch layer; all searching happens in the engine.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -653,10 +654,11 @@ int nl_skin_stats(const nl_skin* s, int64_t* updates, int64_t* rebuilds, double*
 
 void nl_skin_free(nl_skin* s) { delete s; }
 
-int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
-                   const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
-                   const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
-                   int global_index, int64_t capacity, nl_plan** out) {
+static int plan_create_opts(const double* d_positions, size_t n, const int64_t* sys_sizes,
+                            size_t n_sys, const double* cells, const uint8_t* has_cell,
+                            const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs,
+                            int disjoint, int multi, unsigned options, int64_t capacity,
+                            nl_plan** out) {
   if (!out) {
     nl::set_error("output pointer is NULL");
     return 1;
@@ -678,7 +680,11 @@ int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes
   req.cutoffs.assign(cutoffs, cutoffs + n_cutoffs);
   req.multi = multi != 0;
   req.disjoint = disjoint != 0;
-  req.global_index = global_index != 0;
+  req.global_index = (options & NL_LOCAL_INDEX) == 0;
+  req.want_ranges = (options & NL_RANGES) != 0;
+  req.want_dist = (options & NL_DIST) != 0;
+  req.want_disp = (options & NL_DISP) != 0;
+  req.full = (options & NL_FULL) != 0;
   nl_plan* p = new nl_plan();
   p->eng = nl::engine_for_current_device();
   int rc;
@@ -691,6 +697,74 @@ int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes
     return rc;
   }
   *out = p;
+  return 0;
+}
+
+int nl_plan_create(const double* d_positions, size_t n, const int64_t* sys_sizes, size_t n_sys,
+                   const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                   const double* cutoffs, size_t n_cutoffs, int disjoint, int multi,
+                   int global_index, int64_t capacity, nl_plan** out) {
+  return plan_create_opts(d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs,
+                          n_cutoffs, disjoint, multi, global_index ? 0u : NL_LOCAL_INDEX,
+                          capacity, out);
+}
+
+int nl_plan_create_ex(const double* d_positions, size_t n, const int64_t* sys_sizes,
+                      size_t n_sys, const double* cells, const uint8_t* has_cell,
+                      const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs, int disjoint,
+                      int multi, unsigned options, int64_t capacity, nl_plan** out) {
+  return plan_create_opts(d_positions, n, sys_sizes, n_sys, cells, has_cell, pbc, cutoffs,
+                          n_cutoffs, disjoint, multi, options, capacity, out);
+}
+
+int nl_plan_array(const nl_plan* p, size_t list, int what, const void** ptr) {
+  if (!p || list >= (size_t)p->p.pipe().K || !ptr) {
+    nl::set_error("invalid plan, list index or output pointer");
+    return 1;
+  }
+  const int64_t b = p->p.base((int)list);
+  switch (what) {
+    case 0:
+      *ptr = (const int64_t*)p->p.edges() + 2 * b;
+      return 0;
+    case 1:
+      *ptr = (const int32_t*)p->p.shifts() + 3 * b;
+      return 0;
+    case 2:
+      if (!p->p.dist()) {
+        nl::set_error("distances were not requested (NL_DIST)");
+        return 1;
+      }
+      *ptr = (const double*)p->p.dist() + b;
+      return 0;
+    case 3:
+      if (!p->p.disp()) {
+        nl::set_error("displacements were not requested (NL_DISP)");
+        return 1;
+      }
+      *ptr = (const double*)p->p.disp() + 3 * b;
+      return 0;
+    default:
+      nl::set_error("unknown array (0 edge_index, 1 shift, 2 distance, 3 displacement)");
+      return 1;
+  }
+}
+
+int nl_plan_system_ranges(const nl_plan* p, int64_t* ranges, int32_t* modes) {
+  if (!p) {
+    nl::set_error("invalid plan");
+    return 1;
+  }
+  if (!p->p.has_ranges()) {
+    nl::set_error("per-system ranges were not requested (NL_RANGES)");
+    return 1;
+  }
+  if (p->p.ranges().empty()) {
+    nl::set_error("the plan has not run yet");
+    return 1;
+  }
+  if (ranges) std::copy(p->p.ranges().begin(), p->p.ranges().end(), ranges);
+  if (modes) std::copy(p->p.modes().begin(), p->p.modes().end(), modes);
   return 0;
 }
 

@@ -592,9 +592,10 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
     for (int s = 0; s < S; ++s) {
       res->modes[s] = dd[s].mode;
       for (int q = 0; q < K; ++q) {
+        // the scan's positions run over all lists: relative to list q's start
         const int r = P.rank_of(q);
-        res->ranges[2 * ((size_t)s * K + q)] = rr[2 * ((size_t)s * K + r)];
-        res->ranges[2 * ((size_t)s * K + q) + 1] = rr[2 * ((size_t)s * K + r) + 1];
+        res->ranges[2 * ((size_t)s * K + q)] = rr[2 * ((size_t)s * K + r)] - hb[r];
+        res->ranges[2 * ((size_t)s * K + q) + 1] = rr[2 * ((size_t)s * K + r) + 1] - hb[r];
       }
     }
   }
@@ -756,19 +757,26 @@ Plan::~Plan() {
   if (graph_) (void)hipGraphDestroy(graph_);
   for (void* p : owned_)
     if (p) (void)hipFree(p);
-  if (edges_) (void)hipFree(edges_);
-  if (shifts_) (void)hipFree(shifts_);
+  free_out();
   if (pinned_) (void)hipHostFree(pinned_);
+  if (pin_ranges_) (void)hipHostFree(pin_ranges_);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
+void Plan::free_out() {
+  for (void** p : {&edges_, &shifts_, &dist_, &disp_}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
 int Plan::alloc_out(int64_t cap) {
-  if (edges_) (void)hipFree(edges_);
-  if (shifts_) (void)hipFree(shifts_);
-  edges_ = shifts_ = nullptr;
+  free_out();
   cap_ = std::max<int64_t>(cap, 1);
   if (hipMalloc(&edges_, 16 * (size_t)cap_) != hipSuccess ||
-      hipMalloc(&shifts_, 12 * (size_t)cap_) != hipSuccess) {
+      hipMalloc(&shifts_, 12 * (size_t)cap_) != hipSuccess ||
+      (want_dist_ && hipMalloc(&dist_, 8 * (size_t)cap_) != hipSuccess) ||
+      (want_disp_ && hipMalloc(&disp_, 24 * (size_t)cap_) != hipSuccess)) {
     set_error("out of device memory for the plan's edge lists");
     return 1;
   }
@@ -788,14 +796,42 @@ int Plan::capture() {
   if (e != hipSuccess) return eng_->fail(e, "hipStreamBeginCapture");
   e = pipe_.enqueue_upload(buf_, pinned_, st_);
   if (e == hipSuccess) e = pipe_.enqueue_build_count(buf_, st_);
-  if (e == hipSuccess) e = pipe_.enqueue_fill(buf_, edges_, shifts_, nullptr, cap_, st_);
+  if (e == hipSuccess) e = pipe_.enqueue_fill(buf_, edges_, shifts_, dist_, cap_, st_, disp_);
   if (e == hipSuccess) e = pipe_.enqueue_bounds(buf_, pinned_, st_);
+  if (e == hipSuccess && want_ranges_) {
+    const size_t rb = 16 * (size_t)pipe_.S * pipe_.K;
+    e = launch_sys_ranges(buf_.offsets, buf_.items, pipe_.S, pipe_.K, pipe_.n_items, d_ranges_,
+                          st_);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(pin_ranges_, d_ranges_, rb, hipMemcpyDeviceToHost, st_);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(pin_ranges_ + rb, buf_.sys, sizeof(SysDesc) * pipe_.S,
+                         hipMemcpyDeviceToHost, st_);
+  }
   hipError_t e2 = hipStreamEndCapture(st_, &graph_);
   if (e == hipSuccess) e = e2;
   if (e == hipSuccess) e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
   if (e != hipSuccess) return eng_->fail(e, "graph capture");
   ++recaptures;
   return 0;
+}
+
+void Plan::read_ranges() {
+  if (!want_ranges_) return;
+  const int S = pipe_.S, K = pipe_.K;
+  const int64_t* rr = (const int64_t*)pin_ranges_;
+  const SysDesc* dd = (const SysDesc*)(pin_ranges_ + 16 * (size_t)S * K);
+  ranges_.assign((size_t)2 * S * K, 0);
+  modes_.resize(S);
+  for (int s = 0; s < S; ++s) {
+    modes_[s] = dd[s].mode;
+    for (int q = 0; q < K; ++q) {
+      // the scan's positions run over all lists: relative to list q's start
+      const int r = pipe_.rank_of(q);
+      ranges_[2 * ((size_t)s * K + q)] = rr[2 * ((size_t)s * K + r)] - bounds_[r];
+      ranges_[2 * ((size_t)s * K + q) + 1] = rr[2 * ((size_t)s * K + r) + 1] - bounds_[r];
+    }
+  }
 }
 
 int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
@@ -807,6 +843,9 @@ int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
   }
   const Tuning t = eng->tuning_for(req);
   if (pipe_.prepare(req, t)) return 1;
+  want_dist_ = req.want_dist;
+  want_disp_ = req.want_disp;
+  want_ranges_ = req.want_ranges;
   std::vector<size_t> sz = pipe_.buffer_sizes();
   std::vector<void*> ptr(kNumBuffers, nullptr);
   ptr[0] = const_cast<double*>(req.pos);
@@ -825,6 +864,20 @@ int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
     return 1;
   }
   pipe_.stage_host(pinned_);
+  if (want_ranges_) {
+    const size_t rb = 16 * (size_t)pipe_.S * pipe_.K;
+    if (hipMalloc((void**)&d_ranges_, std::max<size_t>(rb, 16)) != hipSuccess) {
+      set_error("out of device memory for the plan");
+      return 1;
+    }
+    owned_.push_back(d_ranges_);
+    if (hipHostMalloc((void**)&pin_ranges_, rb + sizeof(SysDesc) * pipe_.S,
+                      hipHostMallocDefault) != hipSuccess) {
+      pin_ranges_ = nullptr;
+      set_error("out of pinned host memory");
+      return 1;
+    }
+  }
   if (hipStreamCreateWithFlags(&st_, hipStreamNonBlocking) != hipSuccess) {
     set_error("stream creation failed");
     return 1;
@@ -862,6 +915,7 @@ int Plan::run() {
     total_ = hb[pipe_.K];
   }
   bounds_.assign(hb, hb + pipe_.K + 1);
+  read_ranges();
   return 0;
 }
 

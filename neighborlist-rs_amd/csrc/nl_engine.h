@@ -252,12 +252,21 @@ class Plan {
   const Pipeline& pipe() const { return pipe_; }
   void* edges() const { return edges_; }
   void* shifts() const { return shifts_; }
+  void* dist() const { return dist_; }  // FP64 |r_ij| (want_dist), else null
+  void* disp() const { return disp_; }  // FP64 r_ij (want_disp), else null
+  bool has_ranges() const { return want_ranges_; }
+  // per-system edge ranges of the last run, input list order, as
+  // nl_result_system_ranges (want_ranges)
+  const std::vector<int64_t>& ranges() const { return ranges_; }
+  const std::vector<int32_t>& modes() const { return modes_; }
   hipStream_t stream() const { return st_; }
   int replays = 0, recaptures = 0;
 
  private:
   int capture();
   int alloc_out(int64_t cap);
+  void free_out();
+  void read_ranges();
   Engine* eng_ = nullptr;
   Pipeline pipe_;
   Buffers buf_;
@@ -268,6 +277,13 @@ class Plan {
   hipGraphExec_t exec_ = nullptr;
   void* edges_ = nullptr;
   void* shifts_ = nullptr;
+  void* dist_ = nullptr;
+  void* disp_ = nullptr;
+  bool want_dist_ = false, want_disp_ = false, want_ranges_ = false;
+  int64_t* d_ranges_ = nullptr;  // device: [S][K_sorted][2]
+  char* pin_ranges_ = nullptr;   // pinned: the ranges, then the S descriptors
+  std::vector<int64_t> ranges_;
+  std::vector<int32_t> modes_;
   int64_t cap_ = 0, total_ = 0;
   std::vector<int64_t> bounds_;
 };

@@ -400,3 +400,4 @@ from .serving import Server  # noqa: E402  (continuous-batching server)
 from .tensors import (build_neighborlists_torch, build_neighborlists_multi_torch,  # noqa: E402
                       build_neighborlists_batch_torch, build_neighborlists_batch_multi_torch)
 from .md import DynamicNeighborManager  # noqa: E402  (skin lists for MD)
+from .plans import NeighborPlan  # noqa: E402  (graph-captured searches)

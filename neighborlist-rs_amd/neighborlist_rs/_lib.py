@@ -82,6 +82,11 @@ _SIGS = {
     "nl_plan_copy": (C.c_int, [C.c_void_p, C.c_size_t, i64p, i64p, i32p]),
     "nl_plan_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p]),
     "nl_plan_free": (None, [C.c_void_p]),
+    "nl_plan_create_ex": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp,
+                                    C.c_size_t, C.c_int, C.c_int, C.c_uint, C.c_int64,
+                                    C.POINTER(C.c_void_p)]),
+    "nl_plan_array": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
+    "nl_plan_system_ranges": (C.c_int, [C.c_void_p, i64p, i32p]),
     "nl_server_create": (C.c_void_p, [C.c_int, C.c_int64, C.c_int64, C.c_int64]),
     "nl_server_submit": (C.c_int64, [C.c_void_p, dp, C.c_size_t, dp, u8p, C.c_double]),
     "nl_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
