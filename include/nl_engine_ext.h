@@ -188,6 +188,19 @@ int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t*
                     int64_t* max_batch, int64_t* refills);
 /* waits for every submitted request, stops the slot threads, frees the buffers */
 void nl_server_free(nl_server* s);
+/* A server whose requests all get the outputs in `options` (NL_FULL: full
+ * lists; NL_DIST: FP64 |r_ij|; NL_DISP: FP64 r_ij, as nl_search_ex).  The
+ * edge estimate of a full-list request is twice that of its half list. */
+nl_server* nl_server_create_ex(int slots, int64_t max_atoms, int64_t max_systems,
+                               int64_t edge_budget, unsigned options);
+/* nl_server_take and nl_server_result_device with the distances (M float64)
+ * and displacements (M x 3 float64) too; requesting either from a server
+ * created without it is an error, NULL skips it. */
+int nl_server_take_ex(nl_server* s, int64_t id, int64_t* edge_i, int64_t* edge_j,
+                      int32_t* shifts, double* dist, double* disp);
+int nl_server_result_arrays(nl_server* s, int64_t id, const int64_t** edge_i,
+                            const int64_t** edge_j, const int32_t** shifts, const double** dist,
+                            const double** disp);
 
 /* ---- skin lists for molecular dynamics -------------------------------------
  * The dynamic manager of the reference's specification (spec.md:41-48,

@@ -880,6 +880,15 @@ int nl_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hip
 
 nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
                             int64_t edge_budget) {
+  return nl_server_create_ex(slots, max_atoms, max_systems, edge_budget, 0u);
+}
+
+nl_server* nl_server_create_ex(int slots, int64_t max_atoms, int64_t max_systems,
+                               int64_t edge_budget, unsigned options) {
+  if (options & ~(NL_FULL | NL_DIST | NL_DISP)) {
+    nl::set_error("server options: NL_FULL, NL_DIST and NL_DISP only");
+    return nullptr;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     nl::set_error("no HIP device available (libnlmi355 runs on the GPU only)");
@@ -890,7 +899,8 @@ nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
     return nullptr;
   }
   return new nl_server{new nl::Server(nl::engine_for_current_device(), slots, max_atoms,
-                                      max_systems, edge_budget)};
+                                      max_systems, edge_budget, (options & NL_FULL) != 0,
+                                      (options & NL_DIST) != 0, (options & NL_DISP) != 0)};
 }
 
 int64_t nl_server_submit(nl_server* s, const double* positions, size_t n, const double* cell,
@@ -935,6 +945,33 @@ int nl_server_result_device(nl_server* s, int64_t id, const int64_t** edge_i,
     return -1;
   }
   return s->s->device_view(id, edge_i, edge_j, shifts);
+}
+
+int nl_server_take_ex(nl_server* s, int64_t id, int64_t* edge_i, int64_t* edge_j,
+                      int32_t* shifts, double* dist, double* disp) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  if ((dist && !s->s->has_dist()) || (disp && !s->s->has_disp())) {
+    nl::set_error("the server was created without NL_DIST / NL_DISP");
+    return -1;
+  }
+  return s->s->take(id, edge_i, edge_j, shifts, dist, disp);
+}
+
+int nl_server_result_arrays(nl_server* s, int64_t id, const int64_t** edge_i,
+                            const int64_t** edge_j, const int32_t** shifts, const double** dist,
+                            const double** disp) {
+  if (!s) {
+    nl::set_error("server pointer is NULL");
+    return -1;
+  }
+  if ((dist && !s->s->has_dist()) || (disp && !s->s->has_disp())) {
+    nl::set_error("the server was created without NL_DIST / NL_DISP");
+    return -1;
+  }
+  return s->s->device_view(id, edge_i, edge_j, shifts, dist, disp);
 }
 
 int nl_server_release(nl_server* s, int64_t id) {

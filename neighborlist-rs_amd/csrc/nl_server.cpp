@@ -63,14 +63,22 @@ int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc)
   return (int64_t)(0.5 * (double)n * per_atom) + 16;
 }
 
-// Device output of one batch: int64 edges [2][total] then int32 shifts [total][3].
+// Device output of one batch: int64 edges [2][cap], int32 shifts [cap][3],
+// then (when the server emits them) FP64 distances [cap] and displacements
+// [cap][3].  Every block of a server has the same layout.
 struct OutBlock {
   char* dev = nullptr;
   int64_t cap = 0;    // edges the block can hold
   int64_t total = 0;  // edges of the batch written into it
   int refs = 0;
+  bool dist = false, disp = false;
+  static size_t bytes_per_edge(bool dist, bool disp) { return 28 + (dist ? 8 : 0) + (disp ? 24 : 0); }
   int64_t* edges() const { return (int64_t*)dev; }
   int32_t* shifts() const { return (int32_t*)(dev + 16 * (size_t)cap); }
+  double* distances() const { return dist ? (double*)(dev + 28 * (size_t)cap) : nullptr; }
+  double* displacements() const {
+    return disp ? (double*)(dev + (28 + (dist ? 8 : 0)) * (size_t)cap) : nullptr;
+  }
 };
 
 struct SrvReq {
@@ -139,8 +147,14 @@ static bool grow_dev(void** p, size_t* cap, size_t bytes) {
 }
 
 Server::Server(Engine* eng, int slots, int64_t max_atoms, int64_t max_systems,
-               int64_t edge_budget)
-    : eng_(eng), max_atoms_(max_atoms), max_systems_(max_systems), budget_(edge_budget) {
+               int64_t edge_budget, bool full, bool dist, bool disp)
+    : eng_(eng),
+      max_atoms_(max_atoms),
+      max_systems_(max_systems),
+      budget_(edge_budget),
+      full_(full),
+      dist_(dist),
+      disp_(disp) {
   (void)hipSetDevice(eng->device());
   slots = std::max(1, std::min(8, slots));
   for (int i = 0; i < slots; ++i) {
@@ -212,7 +226,7 @@ int64_t Server::submit(const double* pos, int64_t n, const double* h, const uint
     for (int k = 0; k < 3; ++k) r->pbc[k] = pbc ? (pbc[k] != 0) : 1;
   }
   r->cutoff = cutoff;
-  r->est = estimate_edges(pos, n, h, cutoff);
+  r->est = estimate_edges(pos, n, h, cutoff) * (full_ ? 2 : 1);
   std::lock_guard<std::mutex> g(mu_);
   r->id = next_id_++;
   r->t_submit = now_s();
@@ -263,7 +277,8 @@ int Server::info(int64_t id, int64_t* n_edges, double* latency_s, int* batch_siz
   return 0;
 }
 
-int Server::device_view(int64_t id, const int64_t** ei, const int64_t** ej, const int32_t** sh) {
+int Server::device_view(int64_t id, const int64_t** ei, const int64_t** ej, const int32_t** sh,
+                        const double** dist, const double** disp) {
   std::lock_guard<std::mutex> g(mu_);
   SrvReq* r = find_done(id);
   if (!r) return -1;
@@ -271,6 +286,8 @@ int Server::device_view(int64_t id, const int64_t** ei, const int64_t** ej, cons
   if (ei) *ei = b ? b->edges() + r->first : nullptr;
   if (ej) *ej = b ? b->edges() + b->total + r->first : nullptr;
   if (sh) *sh = b ? b->shifts() + 3 * r->first : nullptr;
+  if (dist) *dist = b && b->dist ? b->distances() + r->first : nullptr;
+  if (disp) *disp = b && b->disp ? b->displacements() + 3 * r->first : nullptr;
   return 0;
 }
 
@@ -288,9 +305,11 @@ int Server::release(int64_t id) {
   return 0;
 }
 
-int Server::take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh) {
+int Server::take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh, double* dist,
+                 double* disp) {
   const int64_t *di = nullptr, *dj = nullptr;
   const int32_t* ds = nullptr;
+  const double *dd = nullptr, *dv = nullptr;
   int64_t m = 0;
   SrvReq* r;
   {
@@ -303,6 +322,8 @@ int Server::take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh) {
       di = r->blk->edges() + r->first;
       dj = r->blk->edges() + r->blk->total + r->first;
       ds = r->blk->shifts() + 3 * r->first;
+      if (r->blk->dist) dd = r->blk->distances() + r->first;
+      if (r->blk->disp) dv = r->blk->displacements() + 3 * r->first;
     }
   }
   hipError_t e = hipSuccess;
@@ -311,6 +332,10 @@ int Server::take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh) {
     if (ei && e == hipSuccess) e = hipMemcpy(ei, di, 8 * (size_t)m, hipMemcpyDeviceToHost);
     if (ej && e == hipSuccess) e = hipMemcpy(ej, dj, 8 * (size_t)m, hipMemcpyDeviceToHost);
     if (sh && e == hipSuccess) e = hipMemcpy(sh, ds, 12 * (size_t)m, hipMemcpyDeviceToHost);
+    if (dist && dd && e == hipSuccess)
+      e = hipMemcpy(dist, dd, 8 * (size_t)m, hipMemcpyDeviceToHost);
+    if (disp && dv && e == hipSuccess)
+      e = hipMemcpy(disp, dv, 24 * (size_t)m, hipMemcpyDeviceToHost);
   }
   std::lock_guard<std::mutex> g(mu_);
   drop_locked(r);
@@ -378,7 +403,10 @@ OutBlock* Server::get_block(int64_t cap) {
   }
   OutBlock* b = new OutBlock();
   b->cap = cap;
-  if (hipMalloc((void**)&b->dev, 28 * (size_t)cap) != hipSuccess) {
+  b->dist = dist_;
+  b->disp = disp_;
+  if (hipMalloc((void**)&b->dev, OutBlock::bytes_per_edge(dist_, disp_) * (size_t)cap) !=
+      hipSuccess) {
     delete b;
     return nullptr;
   }
@@ -406,6 +434,7 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   req.n = n;
   req.pos_device = true;
   req.global_index = false;
+  req.full = full_;
   Tuning t;
   Config& c = config();
   t.cell_div = c.cell_div.load();
@@ -451,7 +480,8 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   if (e == hipSuccess) e = s.P.enqueue_upload(s.B, s.pin, s.st);
   if (e == hipSuccess) e = s.P.enqueue_build_count(s.B, s.st);
   if (e == hipSuccess)
-    e = s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), nullptr, s.blk->cap, s.st);
+    e = s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), s.blk->distances(), s.blk->cap,
+                         s.st, s.blk->displacements());
   if (e == hipSuccess)
     e = launch_sys_ranges(s.B.offsets, s.B.items, (int)S, 1, s.P.n_items, s.dranges, s.st);
   if (e == hipSuccess)
@@ -485,8 +515,8 @@ int Server::on_counted(Slot& s) {
     fail_slot(s, "out of device memory for the batch's edge lists");
     return -1;
   }
-  hipError_t e =
-      s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), nullptr, s.blk->cap, s.st);
+  hipError_t e = s.P.enqueue_fill(s.B, s.blk->edges(), s.blk->shifts(), s.blk->distances(),
+                                  s.blk->cap, s.st, s.blk->displacements());
   if (e == hipSuccess) e = hipEventRecord(s.ev, s.st);
   if (e != hipSuccess) {
     fail_slot(s, std::string("HIP error in the server: ") + hipGetErrorString(e));

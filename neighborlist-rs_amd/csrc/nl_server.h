@@ -19,7 +19,10 @@ int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc)
 
 class Server {
  public:
-  Server(Engine* eng, int slots, int64_t max_atoms, int64_t max_systems, int64_t edge_budget);
+  // full: full (directed) lists; dist / disp: FP64 |r_ij| / r_ij per edge
+  // (the same outputs for every request of this server)
+  Server(Engine* eng, int slots, int64_t max_atoms, int64_t max_systems, int64_t edge_budget,
+         bool full = false, bool dist = false, bool disp = false);
   ~Server();
   // returns the request id (> 0) or -1 with the thread's error message set
   int64_t submit(const double* pos, int64_t n, const double* h, const uint8_t* pbc,
@@ -27,9 +30,14 @@ class Server {
   int wait(int64_t id, double timeout_s);  // 0 done, 1 timed out, -1 failed
   int info(int64_t id, int64_t* n_edges, double* latency_s, int* batch_size);
   // device addresses of a completed request's lists (valid until release/take)
-  int device_view(int64_t id, const int64_t** ei, const int64_t** ej, const int32_t** sh);
+  int device_view(int64_t id, const int64_t** ei, const int64_t** ej, const int32_t** sh,
+                  const double** dist = nullptr, const double** disp = nullptr);
   int release(int64_t id);
-  int take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh);  // copies out, then releases
+  // copies out (any pointer may be null), then releases
+  int take(int64_t id, int64_t* ei, int64_t* ej, int32_t* sh, double* dist = nullptr,
+           double* disp = nullptr);
+  bool has_dist() const { return dist_; }
+  bool has_disp() const { return disp_; }
   void stats(int64_t* completed, int64_t* batches, int64_t* max_inflight, int64_t* max_batch,
              int64_t* refills);
 
@@ -47,6 +55,7 @@ class Server {
 
   Engine* eng_;
   int64_t max_atoms_, max_systems_, budget_;
+  bool full_, dist_, disp_;
   std::vector<Slot*> slots_;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;

@@ -98,6 +98,11 @@ _SIGS = {
     "nl_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "nl_server_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p, i64p]),
     "nl_server_free": (None, [C.c_void_p]),
+    "nl_server_create_ex": (C.c_void_p, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint]),
+    "nl_server_take_ex": (C.c_int, [C.c_void_p, C.c_int64, i64p, i64p, i32p, dp, dp]),
+    "nl_server_result_arrays": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "nl_skin_create": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp, dp,
                                  C.c_size_t, C.c_int, C.c_int, C.c_uint, C.c_void_p,
                                  C.POINTER(C.c_void_p)]),

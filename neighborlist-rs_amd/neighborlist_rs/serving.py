@@ -6,6 +6,12 @@ inputs (a PyCell or None, an (N, 3) float64 array, one cutoff) from any
 number of threads, batches whatever is queued into ragged batches on the GPU
 (several in flight on separate HIP streams) and hands each request back the
 same dict build_neighborlists would return for it.
+
+Options beyond the reference, the same for every request of a server:
+``full=True`` (full lists: every pair in both directions, (i, j, S) then
+(j, i, -S)), ``distances=True`` (adds "distance", the FP64 |r_ij|) and
+``displacements=True`` (adds "displacement", the FP64 r_ij = r_j + S H^T -
+r_i), as in the torch search functions.
 """
 import ctypes as C
 
@@ -23,11 +29,15 @@ class Server:
     max_systems  requests per batch
     edge_budget  estimated edges of all in-flight batches (admission control;
                  a request alone is always admitted)
+    full, distances, displacements   outputs of every request (see above)
     """
 
-    def __init__(self, slots=3, max_atoms=1 << 20, max_systems=1024, edge_budget=1 << 28):
-        self._p = lib.nl_server_create(int(slots), int(max_atoms), int(max_systems),
-                                       int(edge_budget))
+    def __init__(self, slots=3, max_atoms=1 << 20, max_systems=1024, edge_budget=1 << 28, *,
+                 full=False, distances=False, displacements=False):
+        self._dist, self._disp = bool(distances), bool(displacements)
+        opts = (16 if full else 0) | (8 if distances else 0) | (32 if displacements else 0)
+        self._p = lib.nl_server_create_ex(int(slots), int(max_atoms), int(max_systems),
+                                          int(edge_budget), opts)
         if not self._p:
             raise RuntimeError(_lib.last_error())
 
@@ -78,16 +88,26 @@ class Server:
 
     def result(self, rid, timeout=None):
         """Blocks until done; returns {"edge_index": (2, M) int64, "shift": (M, 3) int32}
-        and forgets the request."""
+        (plus "distance" (M,) and "displacement" (M, 3) float64 when the server
+        emits them) and forgets the request."""
         if not self.wait(rid, timeout):
             raise TimeoutError(f"request {rid} not done after {timeout} s")
         m, _, _ = self.info(rid)
         ei = np.empty((2, m), np.int64)
         sh = np.empty((m, 3), np.int32)
-        if lib.nl_server_take(self._p, int(rid), ei[0].ctypes.data_as(_lib.i64p),
-                              ei[1].ctypes.data_as(_lib.i64p), sh.ctypes.data_as(_lib.i32p)):
+        d = np.empty(m) if self._dist else None
+        v = np.empty((m, 3)) if self._disp else None
+        if lib.nl_server_take_ex(self._p, int(rid), ei[0].ctypes.data_as(_lib.i64p),
+                                 ei[1].ctypes.data_as(_lib.i64p), sh.ctypes.data_as(_lib.i32p),
+                                 None if d is None else d.ctypes.data_as(_lib.dp),
+                                 None if v is None else v.ctypes.data_as(_lib.dp)):
             raise RuntimeError(_lib.last_error())
-        return {"edge_index": ei, "shift": sh}
+        out = {"edge_index": ei, "shift": sh}
+        if d is not None:
+            out["distance"] = d
+        if v is not None:
+            out["displacement"] = v
+        return out
 
     def result_device(self, rid, timeout=None):
         """Blocks until done; returns (edges, addr_i, addr_j, addr_shift): device

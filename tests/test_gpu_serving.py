@@ -138,3 +138,107 @@ def test_serving_device_results(nr, oracle):
                 srv.release(rid)
             want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
             assert_same_edges((ei[0], ei[1], sh), want, f"request {k}")
+
+
+def _mirror(ei, ej, s):
+    keep = ei != ej
+    return (np.concatenate([ei, ej[keep]]), np.concatenate([ej, ei[keep]]),
+            np.concatenate([s, -s[keep]]))
+
+
+def _sorted_all(ei, ej, s, d, v):
+    o = np.lexsort((s[:, 2], s[:, 1], s[:, 0], ej, ei))
+    return ei[o], ej[o], s[o], d[o], v[o]
+
+
+def _torch_search(nr, p, H, rc):
+    import torch
+    t = nr.build_neighborlists_torch(_cell(nr, H), torch.from_numpy(np.ascontiguousarray(p)).cuda(),
+                                     rc, full=True, distances=True, displacements=True)
+    c = lambda x: x.detach().cpu().numpy()
+    return (c(t["edge_index"][0]), c(t["edge_index"][1]), c(t["shift"]), c(t["distance"]),
+            c(t["displacement"]))
+
+
+def test_serving_full_lists_distances_displacements(nr, oracle):
+    """A server created with full lists, distances and displacements: every
+    request equals the oracle mirrored, and the torch search with the same
+    options edge for edge (distance and displacement bit for bit); alone and
+    batched give the same arrays, order included; a refill (short estimate)
+    rewrites every array."""
+    reqs = _trace(seed=19, n=20, nmax=2500)
+    opts = dict(full=True, distances=True, displacements=True)
+    _print_case("serving full+dist+disp alone", len(reqs))
+    alone, _, _ = _serve(nr, reqs, slots=1, max_systems=1, **opts)
+    _print_case("serving full+dist+disp mix")
+    mix, info, st = _serve(nr, reqs, slots=3, max_atoms=20000, max_systems=64, **opts)
+    assert st["max_batch"] > 1, st
+    for k, ((_, p, H, rc), a, m) in enumerate(zip(reqs, alone, mix)):
+        what = f"request {k}: n={p.shape[0]} rc={rc:.3f} cell={H is not None}"
+        for key in ("edge_index", "shift", "distance", "displacement"):
+            np.testing.assert_array_equal(a[key], m[key], err_msg=what + " " + key)
+        ei, ej = m["edge_index"]
+        sh, d, v = m["shift"], m["distance"], m["displacement"]
+        assert info[k][0] == ei.size
+        # mirror layout and geometry
+        np.testing.assert_array_equal(ei[0::2], ej[1::2], err_msg=what)
+        np.testing.assert_array_equal(sh[0::2], -sh[1::2], err_msg=what)
+        np.testing.assert_array_equal(v[0::2], -v[1::2], err_msg=what)
+        np.testing.assert_array_equal(d, np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) +
+                                                 v[:, 2] * v[:, 2]), err_msg=what)
+        want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
+        assert_same_edges((ei, ej, sh), _mirror(*want), what)
+        for x, y in zip(_sorted_all(ei, ej, sh, d, v), _sorted_all(*_torch_search(nr, p, H, rc))):
+            np.testing.assert_array_equal(x, y, err_msg=what + " vs torch search")
+
+    rng = np.random.default_rng(4)
+    p = np.vstack([rng.uniform(0.0, 6.0, size=(500, 3)), [[800.0, 800.0, 800.0]]])
+    _print_case("serving full+dist+disp refill", p.shape)
+    with nr.Server(slots=1, **opts) as srv:
+        r = srv.result(srv.submit(None, p, 5.0), 60.0)
+        assert srv.stats()["refills"] >= 1
+    ei, ej = r["edge_index"]
+    assert_same_edges((ei, ej, r["shift"]), _mirror(*oracle.search_single(p, None, None, 5.0)),
+                      "refill")
+    for x, y in zip(_sorted_all(ei, ej, r["shift"], r["distance"], r["displacement"]),
+                    _sorted_all(*_torch_search(nr, p, None, 5.0))):
+        np.testing.assert_array_equal(x, y, err_msg="refill vs torch search")
+
+
+def test_serving_result_arrays(nr, oracle):
+    import ctypes as C
+    from neighborlist_rs import _lib
+    lib = _lib.lib
+    reqs = _trace(seed=23, n=6, nmax=1500)
+    _print_case("serving result arrays", len(reqs))
+    with nr.Server(slots=2, distances=True) as srv:
+        ids = [srv.submit(_cell(nr, H), p, rc) for (_, p, H, rc) in reqs]
+        for k, rid in enumerate(ids):
+            _, p, H, rc = reqs[k]
+            assert srv.wait(rid, 60.0)
+            m, _, _ = srv.info(rid)
+            ptr = [C.c_void_p() for _ in range(5)]
+            # displacements were not requested
+            assert lib.nl_server_result_arrays(srv._p, rid, *[C.byref(x) for x in ptr]) != 0
+            assert "NL_DISP" in _lib.last_error()
+            assert lib.nl_server_result_arrays(srv._p, rid, *[C.byref(x) for x in ptr[:4]],
+                                               None) == 0
+            ei = np.empty((2, m), np.int64)
+            sh = np.empty((m, 3), np.int32)
+            d = np.empty(m)
+            if m:
+                for dst, src, nb in ((ei[0], ptr[0], 8 * m), (ei[1], ptr[1], 8 * m),
+                                     (sh, ptr[2], 12 * m), (d, ptr[3], 8 * m)):
+                    assert lib.nl_memcpy_d2h(dst.ctypes.data_as(C.c_void_p), src, nb) == 0
+            srv.release(rid)
+            want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
+            assert_same_edges((ei[0], ei[1], sh), want, f"request {k}")
+            import torch
+            t = nr.build_neighborlists_torch(_cell(nr, H), torch.from_numpy(p).cuda(), rc,
+                                             distances=True)
+            c = lambda x: x.detach().cpu().numpy()
+            o1 = np.lexsort((sh[:, 2], sh[:, 1], sh[:, 0], ei[1], ei[0]))
+            ts = c(t["shift"])
+            te = c(t["edge_index"])
+            o2 = np.lexsort((ts[:, 2], ts[:, 1], ts[:, 0], te[1], te[0]))
+            np.testing.assert_array_equal(d[o1], c(t["distance"])[o2], err_msg=f"request {k}")
