@@ -254,6 +254,14 @@ int nl_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int nl_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int nl_device_synchronize(void);
 
+/* Page-locked host memory for lists copied to host arrays (no counterpart in
+ * the reference, whose lists never leave host memory): a device-to-host copy
+ * into it runs at the link's DMA rate.  nl_host_alloc returns NULL when the
+ * pool's limit (8 GiB handed out) is reached or the runtime refuses; the
+ * caller then uses ordinary memory.  Blocks are recycled by nl_host_free. */
+void* nl_host_alloc(size_t bytes);
+void nl_host_free(void* p);
+
 /* device selection for subsequent calls on this thread */
 int nl_set_device(int device);
 int nl_device_count(void);

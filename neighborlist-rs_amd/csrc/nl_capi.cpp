@@ -4,8 +4,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nl_api.h"
@@ -300,6 +304,11 @@ int nl_result_copy(const nl_result* r, size_t list, int64_t* edge_i, int64_t* ed
   hipStream_t st = R->eng->stream();
   const int64_t* e = (const int64_t*)R->edges;
   hipError_t err = hipSuccess;
+  if (edge_i && edge_j == edge_i + m) {
+    // the (2, M) layout on both sides: one copy
+    err = hipMemcpyAsync(edge_i, e + 2 * b, 16 * m, hipMemcpyDeviceToHost, st);
+    edge_i = edge_j = nullptr;
+  }
   if (edge_i && err == hipSuccess)
     err = hipMemcpyAsync(edge_i, e + 2 * b, 8 * m, hipMemcpyDeviceToHost, st);
   if (edge_j && err == hipSuccess)
@@ -877,6 +886,70 @@ int nl_memcpy_d2d(void* dst, const void* src, size_t bytes) {
   return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
 }
 int nl_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+// Pinned host blocks for lists delivered to host arrays: a device-to-host
+// copy into page-locked memory runs at the link's DMA rate, into pageable
+// memory through the runtime's staging buffer at a fraction of it.  Blocks
+// come in power-of-two sizes from 1 MiB; freed blocks are kept for reuse up
+// to kPinnedCache bytes, and at most kPinnedLive bytes are handed out at once
+// (beyond it nl_host_alloc returns NULL and callers use pageable memory).
+namespace {
+constexpr size_t kPinnedCache = size_t(1) << 30;
+constexpr size_t kPinnedLive = size_t(8) << 30;
+struct PinnedPool {
+  std::mutex mu;
+  std::multimap<size_t, void*> idle;           // size -> block
+  std::unordered_map<void*, size_t> size_of;  // every block handed out
+  size_t cached = 0, live = 0;
+};
+PinnedPool& pinned_pool() {
+  static PinnedPool* pool = new PinnedPool();  // lives until the process ends
+  return *pool;
+}
+}  // namespace
+
+void* nl_host_alloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  size_t cls = size_t(1) << 20;
+  while (cls < bytes) cls <<= 1;
+  PinnedPool& P = pinned_pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  // the most recently freed block of the class first
+  auto it = P.idle.upper_bound(cls);
+  void* p = nullptr;
+  if (it != P.idle.begin() && std::prev(it)->first == cls) {
+    it = std::prev(it);
+    p = it->second;
+    P.idle.erase(it);
+    P.cached -= cls;
+  } else {
+    if (P.live + cls > kPinnedLive) return nullptr;
+    if (hipHostMalloc(&p, cls, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  P.size_of[p] = cls;
+  P.live += cls;
+  return p;
+}
+
+void nl_host_free(void* p) {
+  if (!p) return;
+  PinnedPool& P = pinned_pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  auto it = P.size_of.find(p);
+  if (it == P.size_of.end()) return;  // not a block of this pool
+  const size_t cls = it->second;
+  P.size_of.erase(it);
+  P.live -= cls;
+  if (P.cached + cls <= kPinnedCache) {
+    P.idle.emplace(cls, p);
+    P.cached += cls;
+  } else {
+    (void)hipHostFree(p);
+  }
+}
 
 nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
                             int64_t edge_budget) {

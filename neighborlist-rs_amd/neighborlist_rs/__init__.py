@@ -136,6 +136,44 @@ def _empty():
     return {"edge_index": np.zeros((2, 0), np.int64), "shift": np.zeros((0, 3), np.int32)}
 
 
+_PINNED_MIN = 1 << 20  # bytes; smaller lists are copied into ordinary numpy memory
+
+
+class _Pinned:
+    """Owner of a page-locked block of the engine's pool: the block goes back
+    to the pool when the last array over it is gone."""
+    __slots__ = ("p",)
+
+    def __init__(self, p):
+        self.p = p
+
+    def __del__(self):
+        free = getattr(lib, "nl_host_free", None)  # None at interpreter exit
+        if free is not None and self.p:
+            free(C.c_void_p(self.p))
+            self.p = None
+
+
+def _host_block(specs):
+    """numpy arrays for [(shape, dtype), ...], back to back in one page-locked
+    block (a device-to-host copy into it runs at the link's DMA rate), or in
+    ordinary memory below _PINNED_MIN bytes or when the pool has no block.
+    List 8-byte types first: every array then starts aligned."""
+    sizes = [int(np.prod(shape)) * np.dtype(dt).itemsize for shape, dt in specs]
+    total = sum(sizes)
+    p = lib.nl_host_alloc(total) if total >= _PINNED_MIN else None
+    if not p:
+        return [np.empty(shape, dt) for shape, dt in specs]
+    buf = (C.c_char * total).from_address(p)
+    buf._owner = _Pinned(p)
+    out, off = [], 0
+    for (shape, dt), nb in zip(specs, sizes):
+        out.append(np.frombuffer(buf, dt, count=nb // np.dtype(dt).itemsize,
+                                 offset=off).reshape(shape))
+        off += nb
+    return out
+
+
 class _Result:
     def __init__(self):
         self.ptr = C.c_void_p()
@@ -147,8 +185,7 @@ class _Result:
 
     def list(self, q):
         m = int(lib.nl_result_num_edges(self.ptr, q))
-        ei = np.empty((2, m), np.int64)
-        sh = np.empty((m, 3), np.int32)
+        ei, sh = _host_block([((2, m), np.int64), ((m, 3), np.int32)])
         if m:
             _check(lib.nl_result_copy(self.ptr, q, ei[0].ctypes.data_as(_lib.i64p),
                                       ei[1].ctypes.data_as(_lib.i64p),

@@ -116,6 +116,8 @@ _SIGS = {
     "nl_cell_inverse": (C.c_int, [dp, dp]),
     "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
     "nl_device_malloc": (C.c_void_p, [C.c_size_t]),
+    "nl_host_alloc": (C.c_void_p, [C.c_size_t]),
+    "nl_host_free": (None, [C.c_void_p]),
     "nl_device_free": (C.c_int, [C.c_void_p]),
     "nl_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "nl_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -92,11 +92,16 @@ class Server:
         emits them) and forgets the request."""
         if not self.wait(rid, timeout):
             raise TimeoutError(f"request {rid} not done after {timeout} s")
+        from . import _host_block
+
         m, _, _ = self.info(rid)
-        ei = np.empty((2, m), np.int64)
-        sh = np.empty((m, 3), np.int32)
-        d = np.empty(m) if self._dist else None
-        v = np.empty((m, 3)) if self._disp else None
+        specs = [((2, m), np.int64)]
+        specs += [((m,), np.float64)] if self._dist else []
+        specs += [((m, 3), np.float64)] if self._disp else []
+        arrs = _host_block(specs + [((m, 3), np.int32)])
+        ei, sh = arrs[0], arrs[-1]
+        d = arrs[1] if self._dist else None
+        v = arrs[1 + int(self._dist)] if self._disp else None
         if lib.nl_server_take_ex(self._p, int(rid), ei[0].ctypes.data_as(_lib.i64p),
                                  ei[1].ctypes.data_as(_lib.i64p), sh.ctypes.data_as(_lib.i32p),
                                  None if d is None else d.ctypes.data_as(_lib.dp),
