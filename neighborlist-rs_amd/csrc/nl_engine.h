@@ -63,6 +63,9 @@ struct Request {
   int parts = 0;                // 0: tuned / engine default
 };
 
+// Requests below this many atoms are not autotuned (see Engine::tuning_for).
+constexpr int64_t kTuneMinAtoms = 2048;
+
 struct Tuning {
   int cell_div = 3, use_fp32 = 0, group = 4, parts = 1;
 };

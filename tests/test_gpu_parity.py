@@ -4,6 +4,8 @@ Bit-exact bar: after the canonical sort on (i, j, S), edge_index and shift
 arrays are identical to the oracle's (which restates the reference and is
 pinned by the reference's own known answers, see test_oracle_reference.py).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -61,6 +63,8 @@ def test_large_fixture_logged_counts(nr, name, rc, count):
 def test_configs_vs_oracle(nr, oracle, cell_div, fp32):
     from neighborlist_rs import workloads as W
     from neighborlist_rs._lib import lib
+    was = (lib.nl_get_autotune(), lib.nl_get_cell_div(), lib.nl_get_fp32_filter())
+    lib.nl_set_autotune(0)  # the settings under test reach every request
     lib.nl_set_cell_div(cell_div)
     lib.nl_set_fp32_filter(fp32)
     try:
@@ -75,8 +79,9 @@ def test_configs_vs_oracle(nr, oracle, cell_div, fp32):
             got = _gpu_single(nr, p, H, [True] * 3, rc)
             assert_same_edges(got, want, f"{name} rc={rc} div={cell_div} fp32={fp32}")
     finally:
-        lib.nl_set_cell_div(2)
-        lib.nl_set_fp32_filter(0)
+        lib.nl_set_autotune(was[0])
+        lib.nl_set_cell_div(was[1])
+        lib.nl_set_fp32_filter(was[2])
 
 
 def test_c3_rc14_vs_oracle(nr, oracle):
@@ -169,7 +174,6 @@ def test_small_box_many_images(nr, oracle):
 
 def test_distances_fp64(nr, oracle):
     """FP64 |r_ij| emitted with the edges equals a float64 recomputation."""
-    import ctypes as C
     from neighborlist_rs import workloads as W
     from neighborlist_rs._lib import lib, dp, i64p, i32p
     p, H, _ = W.c2_ethanol()
@@ -202,6 +206,10 @@ def test_distances_fp64(nr, oracle):
 def test_work_split_does_not_change_results(nr, oracle, group, parts):
     from neighborlist_rs import workloads as W
     from neighborlist_rs._lib import lib
+    g0, p0 = C.c_int(), C.c_int()
+    lib.nl_get_work_split(C.byref(g0), C.byref(p0))
+    was = lib.nl_get_autotune()
+    lib.nl_set_autotune(0)  # the split under test reaches every request
     lib.nl_set_work_split(group, parts)
     try:
         p, H, _ = W.c2_ethanol()
@@ -218,7 +226,8 @@ def test_work_split_does_not_change_results(nr, oracle, group, parts):
         for k in range(len(cut)):
             assert_same_edges(as_tuple(got[k]), want[k], f"G={group} P={parts} list {k}")
     finally:
-        lib.nl_set_work_split(8, 2)
+        lib.nl_set_work_split(g0.value, p0.value)
+        lib.nl_set_autotune(was)
 
 
 @pytest.mark.parametrize("disjoint", [False, True])
