@@ -186,9 +186,9 @@ def serve_trace(L, n_req, slots, load):
     """Serving trace: Poisson arrivals of single systems (N log-uniform in
     [16, 50000], isolated / orthorhombic / triclinic, rc uniform in [4, 12]).
 
-    Capacity: all requests queued at once (closed loop), results left on the
-    device (the consumer of a served list is a GPU model) and, separately,
-    copied into numpy arrays.  Latency: open loop, Poisson arrivals at `load`
+    Capacity: all requests queued at once (closed loop, median of three
+    runs), results left on the device (the consumer of a served list is a GPU
+    model) and, separately, copied into numpy arrays.  Latency: open loop, Poisson arrivals at `load`
     times the measured capacity; latency = submit -> lists complete on the
     device.  Plain: the same requests one call each through the plain path
     (host input, lists left on the device)."""
@@ -240,8 +240,9 @@ def serve_trace(L, n_req, slots, load):
         return time.perf_counter() - t0, tot
 
     closed(False)  # warm-up: grows slot buffers and the output pool
-    cap_s, served_edges = closed(False)
-    assert served_edges == plain_edges, (served_edges, plain_edges)
+    runs = [closed(False) for _ in range(3)]  # median of three closed loops
+    cap_s, served_edges = sorted(runs)[1]
+    assert all(e == plain_edges for _, e in runs), (runs, plain_edges)
     host_s, host_edges = closed(True)
     assert host_edges == plain_edges
     capacity = n_req / cap_s
@@ -435,7 +436,7 @@ def main():
                     help="'auto' (autotuner) or cell_div,group,parts,fp32 pinned")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the timed batch steps (for rocprofv3)")
-    ap.add_argument("--serve-requests", type=int, default=300,
+    ap.add_argument("--serve-requests", type=int, default=600,
                     help="requests in the serving trace (0: skip serving)")
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-load", type=float, default=0.7,
