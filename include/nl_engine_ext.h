@@ -87,8 +87,10 @@ void nl_set_timing(int on);
 int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* total_ms,
                    int64_t* edges, int64_t* bins);
 
-/* Tunables: GPU bin edge = max cutoff / cell_div (1..4); FP32 filter with
- * FP64 band recheck on the cell-list path (0/1).  Neither changes results. */
+/* Tunables: GPU bin edge = max cutoff / cell_div (1..4), or 0 (the default):
+ * each system takes the most bins per cutoff, up to 4, that still hold six
+ * atoms at its density; FP32 filter with FP64 band recheck on the cell-list
+ * path (0/1).  Neither changes results. */
 void nl_set_cell_div(int d);
 int nl_get_cell_div(void);
 void nl_set_fp32_filter(int on);

@@ -527,7 +527,7 @@ int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* tota
   return 0;
 }
 
-void nl_set_cell_div(int d) { nl::config().cell_div.store(d < 1 ? 1 : (d > 4 ? 4 : d)); }
+void nl_set_cell_div(int d) { nl::config().cell_div.store(d < 0 ? 0 : (d > 4 ? 4 : d)); }
 int nl_get_cell_div(void) { return nl::config().cell_div.load(); }
 void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }
 int nl_get_fp32_filter(void) { return nl::config().use_fp32.load(); }

@@ -62,7 +62,7 @@ struct Params {
   int32_t multi;       // search_*_multi rules
   int32_t bf_threshold_lo;  // brute-force threshold, split into two
   int32_t bf_threshold_hi;  // 32-bit halves (size_t semantics)
-  int32_t cell_div;    // GPU bin edge = rc_max / cell_div
+  int32_t cell_div;    // GPU bin edge = rc_max / cell_div; 0: bins_per_cutoff per system
   int32_t use_fp32;    // FP32 filter + FP64 band recheck on the cell-list path
   int32_t global_index;// add atom_start to emitted indices (batch API)
   int32_t per_sys_rc;  // rc2 holds [n_sys][k] squared cutoffs instead of [k]
@@ -111,6 +111,25 @@ NL_HD void widths3(const double* hinv, double* w) {
 }
 
 NL_HD double norm2(double x, double y, double z) { return (x * x + y * y) + z * z; }
+
+// Bins per cutoff length chosen per system (cell_div 0): the largest of 1..4
+// whose bins still hold kBinAtoms atoms at the system's density n / volume.
+// Finer bins test fewer candidates per pair but add segments and shrink the
+// home groups; on gfx950 the finer grid pays once its bins hold about six
+// atoms (measured on periodic systems of 20,000 atoms at densities 0.03 and
+// 0.1 per cubic angstrom, cutoffs 3 to 14 angstrom).  Host and device use
+// the same rule; the grid never changes a result.
+constexpr double kBinAtoms = 6.0;
+NL_HD int bins_per_cutoff(double rc, double n_atoms, double volume) {
+  const double q = n_atoms * rc * rc * rc;  // atoms per bin at cd = 1, times volume
+  for (int cd = 4; cd > 1; --cd)
+    if (q >= kBinAtoms * (double)(cd * cd * cd) * volume) return cd;
+  return 1;  // also for a NaN or zero volume
+}
+NL_HD double det3(const double* h) {
+  return h[0] * (h[4] * h[8] - h[7] * h[5]) - h[1] * (h[3] * h[8] - h[6] * h[5]) +
+         h[2] * (h[3] * h[7] - h[6] * h[4]);
+}
 
 // Rust `as i32` (saturating, truncating toward zero, NaN -> 0).
 NL_HD int32_t as_i32(double x) {

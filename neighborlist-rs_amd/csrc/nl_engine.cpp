@@ -108,7 +108,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
 
   tune.group = std::max(1, std::min(64, tune.group));
   tune.parts = std::max(1, std::min(16, tune.parts));
-  tune.cell_div = std::max(1, std::min(4, tune.cell_div));
+  tune.cell_div = std::max(0, std::min(4, tune.cell_div));  // 0: per system
   // per-system descriptors (static part), bin reservation
   desc.assign(S, SysDesc{});
   bin_total = 0;
@@ -133,7 +133,10 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
       // k_plan shrinks it to the reservation, which any covering grid allows
       double w[3];
       widths3(inv, w);
-      const double edge = d.rc_max / (double)tune.cell_div;
+      const int cd = tune.cell_div > 0
+                         ? tune.cell_div
+                         : bins_per_cutoff(d.rc_max, (double)d.n_atoms, std::fabs(det3(d.h)));
+      const double edge = d.rc_max / (double)cd;
       int64_t prod = 1;
       for (int k = 0; k < 3; ++k) {
         const double v = std::floor(w[k] / edge);
@@ -679,10 +682,11 @@ Tuning Engine::tuning_for(const Request& req) {
     return best;
   };
   // bin size first (a candidate must beat the incumbent by 2%, so timing
-  // noise does not walk the choice away from the default) ...
+  // noise does not walk the choice away from the default; 0 = each system
+  // by its density) ...
   Tuning best = t;
   double tb = time_one(best, 3);
-  for (int dv : {1, 2, 3, 4}) {
+  for (int dv : {0, 1, 2, 3, 4}) {
     if (dv == t.cell_div) continue;
     Tuning x = best;
     x.cell_div = dv;

@@ -21,7 +21,7 @@ struct Config {
   std::atomic<size_t> bf_threshold{1000};
   std::atomic<size_t> par_threshold{300};
   std::atomic<size_t> stack_threshold{1000};
-  std::atomic<int> cell_div{3};  // GPU bin edge = rc_max / cell_div
+  std::atomic<int> cell_div{0};  // GPU bin edge = rc_max / cell_div; 0: per system (density)
   std::atomic<int> use_fp32{0};  // FP32 filter + FP64 band recheck
   std::atomic<int> group{4};     // home atoms per cell-list work item
   std::atomic<int> parts{1};     // half-stencil slices per home group
@@ -67,7 +67,7 @@ struct Request {
 constexpr int64_t kTuneMinAtoms = 2048;
 
 struct Tuning {
-  int cell_div = 3, use_fp32 = 0, group = 4, parts = 1;
+  int cell_div = 0, use_fp32 = 0, group = 4, parts = 1;
 };
 
 // Device buffers of one launch sequence (owned by an Engine or a Plan).

@@ -173,6 +173,7 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
   __syncthreads();
   if (threadIdx.x != 0) return;
   double h[9];
+  double occupied = 1.0;  // volume the atoms occupy (density for the bin size)
   if (!d->has_cell) {
     const double margin = rc_max + 1.0;  // AUTO_BOX_MARGIN (single.rs:6)
     for (int k = 0; k < 9; ++k) h[k] = 0.0;
@@ -183,10 +184,13 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
         mx = s_mx[k][w] > mx ? s_mx[k][w] : mx;
       }
       h[4 * k] = (mx - mn) + 2.0 * margin;
+      // the bounding box, each side at least half a cutoff (flat clusters)
+      occupied *= fmax(mx - mn, 0.5 * rc_max);
     }
     for (int k = 0; k < 3; ++k) d->pbc[k] = 0;
   } else {
     for (int k = 0; k < 9; ++k) h[k] = d->h[k];
+    occupied = fabs(det3(h));
   }
   double hinv[9];
   if (!inverse3(h, hinv)) {
@@ -226,7 +230,9 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
     return;
   }
   // GPU bin grid: edge ~ rc_max / cell_div, at most bin_cap bins.
-  const double edge = rc_max / (double)p.cell_div;
+  const int cell_div =
+      p.cell_div > 0 ? p.cell_div : bins_per_cutoff(rc_max, (double)n, occupied);
+  const double edge = rc_max / (double)cell_div;
   int nb[3];
   for (int k = 0; k < 3; ++k) {
     double v = floor(w[k] / edge);

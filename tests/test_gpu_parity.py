@@ -58,7 +58,7 @@ def test_large_fixture_logged_counts(nr, name, rc, count):
     assert np.all((ei < ej) | ((ei == ej) & np.any(s != 0, axis=1)))
 
 
-@pytest.mark.parametrize("cell_div", [1, 2, 3])
+@pytest.mark.parametrize("cell_div", [0, 1, 2, 3])
 @pytest.mark.parametrize("fp32", [0, 1])
 def test_configs_vs_oracle(nr, oracle, cell_div, fp32):
     from neighborlist_rs import workloads as W
