@@ -1088,6 +1088,11 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             px = NAN;
           }
           const int oj = F32 ? __float_as_int(c32.w) : __double2loint(c.w);
+          // fill pass, per chunk: the candidate's part of S (s_j + bs) and its
+          // emitted index
+          const int tjx = cj.x + sbx, tjy = cj.y + sby, tjz = cj.z + sbz;
+          const int64_t base_idx = A.global_index ? a0 : 0;
+          const int64_t gj_c = base_idx + oj;
           // home atom i is tested against this lane's candidate iff i < lim
           // (in the home bin without image only the later slots)
           const int lim = cvalid ? (home_seg ? cjs - hb : 64) : 0;
@@ -1181,22 +1186,16 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
               int64_t gi = 0, gj = 0;
               if (FILL) {
                 const int4 hsi = L.homesi[i];
-                const int six = hsi.x, siy = hsi.y, siz = hsi.z;
-                if (mem) {
-                  sx = cj.x - six + sbx;
-                  sy = cj.y - siy + sby;
-                  sz = cj.z - siz + sbz;
-                  int lo_i = oi, hi_i = oj;
-                  if (oj < oi) {
-                    lo_i = oj;
-                    hi_i = oi;
-                    sx = -sx;
-                    sy = -sy;
-                    sz = -sz;
-                  }
-                  gi = A.global_index ? a0 + lo_i : (int64_t)lo_i;
-                  gj = A.global_index ? a0 + hi_i : (int64_t)hi_i;
-                }
+                // S = s_j - s_i + bs in the reference orientation (lower
+                // original index first): negated when the candidate is lower
+                const bool flip = oj < oi;
+                const int sgn = flip ? -1 : 0;  // x -> (x ^ sgn) - sgn negates when flipped
+                sx = ((tjx - hsi.x) ^ sgn) - sgn;
+                sy = ((tjy - hsi.y) ^ sgn) - sgn;
+                sz = ((tjz - hsi.z) ^ sgn) - sgn;
+                const int64_t gi_h = base_idx + oi;  // uniform
+                gi = flip ? gj_c : gi_h;
+                gj = flip ? gi_h : gj_c;
               }
               // displacement in the edge's orientation (lower original index
             // first), recomputed from the operands of d^2 only when asked for
