@@ -64,5 +64,19 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+CPP_TEST = os.path.join(BUILD, "test_api_cpp")
+
+
+def build_cpp_test(src: str, out: str = CPP_TEST) -> str:
+    """A test program over the C++ mirror of the reference's Rust API
+    (include/neighborlist.hpp): host C++ only, linked to the library."""
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I", INC, src,
+           "-L", OUT_DIR, "-lnlmi355", "-Wl,-rpath,$ORIGIN/../neighborlist_rs/lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
