@@ -434,6 +434,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tuning", default="auto",
                     help="'auto' (autotuner) or cell_div,group,parts,fp32 pinned")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="run only the serving trace (for rocprofv3)")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the timed batch steps (for rocprofv3)")
     ap.add_argument("--serve-requests", type=int, default=600,
@@ -469,6 +471,11 @@ def main():
         lib.nl_device_free(C.c_void_p(a))
         lib.nl_device_free(C.c_void_p(b))
         log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB device-to-device")
+
+    if args.only_serving:
+        print(json.dumps({"serving": serve_trace(L, args.serve_requests, args.serve_slots,
+                                                 args.serve_load)}), flush=True)
+        return
 
     # ---- headline: ragged batch, three cutoffs, one pass ------------------
     pos, sizes, cells, cut = W.c4_ragged_batch()

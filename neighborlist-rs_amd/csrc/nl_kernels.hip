@@ -133,7 +133,7 @@ __global__ void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restri
 // cell-list distance uses wrapped positions and the image shift only
 // (search.rs:540-549, 695-697), so any grid with a covering stencil gives
 // the reference's edge set.
-__global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
+__global__ __launch_bounds__(1024) void k_plan(const double* __restrict__ pos,
                                               SysDesc* __restrict__ sys,
                                               float2* __restrict__ band,
                                               const double* __restrict__ rc2, Params p) {
@@ -143,17 +143,29 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
   const int n = d->n_atoms;
   const double rc_max = d->rc_max;
   const double* rc2s = p.per_sys_rc ? rc2 + (int64_t)s * p.k : rc2;
-  __shared__ double s_mn[3][4], s_mx[3][4];
+  // bounding box of an isolated system: 16 waves, four atoms per thread in
+  // flight (a system of 50,000 atoms is 13 rounds of loads, not 200); the
+  // fold skips NaN and does not depend on the order
+  constexpr int kWavesPlan = 16;
+  __shared__ double s_mn[3][kWavesPlan], s_mx[3][kWavesPlan];
   if (!d->has_cell) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const double* q = pos + 3 * (a0 + i);
+    for (int i0 = threadIdx.x; i0 < n; i0 += 4 * blockDim.x) {
+      double v[4][3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double v = q[k];
-        mn[k] = v < mn[k] ? v : mn[k];
-        mx[k] = v > mx[k] ? v : mx[k];
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * (int)blockDim.x;
+        const double* q = pos + 3 * (a0 + (i < n ? i : 0));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[u][k] = i < n ? q[k] : NAN;
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          mn[k] = v[u][k] < mn[k] ? v[u][k] : mn[k];
+          mx[k] = v[u][k] > mx[k] ? v[u][k] : mx[k];
+        }
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -179,7 +191,7 @@ __global__ __launch_bounds__(256) void k_plan(const double* __restrict__ pos,
     for (int k = 0; k < 9; ++k) h[k] = 0.0;
     for (int k = 0; k < 3; ++k) {
       double mn = s_mn[k][0], mx = s_mx[k][0];
-      for (int w = 1; w < 4; ++w) {
+      for (int w = 1; w < kWavesPlan; ++w) {
         mn = s_mn[k][w] < mn ? s_mn[k][w] : mn;
         mx = s_mx[k][w] > mx ? s_mx[k][w] : mx;
       }
@@ -1640,7 +1652,7 @@ hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStr
 hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
                        const Params& p, hipStream_t st) {
   if (p.n_sys <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_plan, dim3(p.n_sys), dim3(256), 0, st, pos, sys, band, rc2, p);
+  hipLaunchKernelGGL(k_plan, dim3(p.n_sys), dim3(1024), 0, st, pos, sys, band, rc2, p);
   return hipGetLastError();
 }
 
