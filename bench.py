@@ -448,7 +448,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(batches and serving traces would shard by system, one process per GPU); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from neighborlist_rs import _lib as L
