@@ -620,6 +620,35 @@ def main():
     assert plain_edges == edges, (plain_edges, edges)
     log(f"plain: {plain_sys_per_s:.1f} systems/s vs batched {sys_per_s:.1f}")
 
+    # ---- single small isolated systems: one launch vs the pipeline -------
+    small_calls = {"workload": "isolated clusters at 0.1 atoms/A^3, rc 6.0, one call each through "
+                               "nl_search_batch_device (positions on the device, lists left there); "
+                               "median host-clock time per call over 200 calls",
+                   "timing": "one_launch = nl_set_small_path(1) (the default), pipeline = "
+                             "nl_set_small_path(0)"}
+    for n_s in (9, 100):
+        ps = np.random.default_rng(n_s).uniform(0.0, (n_s / 0.1) ** (1 / 3), (n_s, 3))
+        dps = DevArray(lib, ps)
+        ss = np.array([n_s], np.int64)
+        row = {}
+        for mode, key in ((1, "one_launch_us"), (0, "pipeline_us")):
+            lib.nl_set_small_path(mode)
+            run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)), np.zeros(1, np.uint8),
+                        np.zeros(3, np.uint8), np.array([6.0]), False)
+            tt = []
+            for _ in range(200):
+                t0 = time.perf_counter()
+                es = run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)),
+                                 np.zeros(1, np.uint8), np.zeros(3, np.uint8), np.array([6.0]),
+                                 False)
+                tt.append(time.perf_counter() - t0)
+            row[key] = round(float(np.median(tt)) * 1e6, 1)
+            row["edges"] = es
+        lib.nl_set_small_path(1)
+        dps.free()
+        small_calls[f"{n_s}_atoms"] = row
+    log(f"small single systems: {small_calls}")
+
     # ---- 10k-atom triclinic PBC at rc 6 (single system) ------------------
     p3, H3, _ = W.c3_triclinic()
     d3 = DevArray(lib, p3)
@@ -699,6 +728,7 @@ def main():
         "systems_per_s": round(sys_per_s, 1),
         "plain_gpu_systems_per_s": round(plain_sys_per_s, 1),
         "batched_over_plain": round(sys_per_s / plain_sys_per_s, 2),
+        "small_single_systems": small_calls,
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 4),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 4)},
         "stage_ms": {"build_and_count": round(float(np.mean(counts)), 4),

@@ -93,6 +93,17 @@ int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* tota
  * path (0/1).  Neither changes results. */
 void nl_set_cell_div(int d);
 int nl_get_cell_div(void);
+/* Single small systems below the brute-force threshold run as one kernel
+ * launch (plan, counts, scan and fill in one block) instead of the
+ * nine-launch pipeline; the lists are byte-identical either way.
+ * 1 (the default): on for systems of at most 128 atoms, isolated or in cells
+ * whose widths leave twice the largest cutoff clear of the minimum-image
+ * bound (where one block beats the pipeline's spread over the chip);
+ * 2: on for every single system of at most 1024 atoms (the kernel's plan
+ * hands cell-list systems back to the pipeline); 0: off.  No counterpart in
+ * the reference (a GPU launch choice). */
+void nl_set_small_path(int mode);
+int nl_get_small_path(void);
 void nl_set_fp32_filter(int on);
 int nl_get_fp32_filter(void);
 /* Work decomposition of the cell-list pair kernels: `group` home atoms of

@@ -24,6 +24,9 @@ constexpr int kScanChainTiles = 32;
 // segmented sort of small systems: atoms per system, digit width
 constexpr int kSegMax = 4096;
 constexpr int kSegBits = 7;
+// single systems of at most this many atoms on the brute-force path: one
+// block plans, counts, scans and fills them in one launch (k_small)
+constexpr int kSmallRows = 1024;
 
 // Search mode chosen per system, exactly as single.rs:60-89 / batch.rs:100-124.
 enum Mode : int32_t {

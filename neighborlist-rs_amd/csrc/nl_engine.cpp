@@ -364,7 +364,7 @@ hipError_t Pipeline::enqueue_bounds(const Buffers& b, char* pin, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Engine
 
-enum { kRangesSlot = kNumBuffers, kSlots };
+enum { kRangesSlot = kNumBuffers, kSmallSlot, kSlots };
 
 Engine::Engine(int device) : device_(device), scratch_(kSlots) {
   (void)hipSetDevice(device_);
@@ -513,6 +513,14 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
   (void)hipSetDevice(device_);
   Pipeline P;
   if (P.prepare(req, t)) return 1;
+  if (small_eligible(req, P)) {
+    const int r = run_small(req, P, res.get());
+    if (r != 2) {
+      if (r == 0) *out = res.release();
+      return r;
+    }
+    // the plan took the cell-list path: the pipeline below decides alike
+  }
   const int S = P.S;
   std::vector<size_t> sz = P.buffer_sizes();
   std::vector<void*> ptr(kNumBuffers);
@@ -620,6 +628,109 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
     times_.bins = P.bin_total;
   }
   *out = res.release();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Single small systems: one launch of k_small instead of the pipeline.  The
+// plan inside the kernel is the pipeline's, so the host only screens: one
+// system below the brute-force threshold, at most kGroup lists and no
+// per-system ranges; by default at most kSmallAuto atoms and, with a cell,
+// widths that leave the cutoff clear of the minimum-image bound (otherwise
+// the pipeline runs at once rather than after a declined launch).  One block
+// does all rows, so the pipeline's spread over the chip wins beyond about
+// 150-300 atoms (measured: 33 vs 80 us at 9 atoms, 46 vs 80 us at 100,
+// 74 vs 79 us at 200 with the minimum-image branch); mode 2 lifts both
+// screens up to the kernel's kSmallRows.
+constexpr int64_t kSmallAuto = 128;
+
+bool Engine::small_eligible(const Request& req, const Pipeline& P) const {
+  const int on = config().small_path.load();
+  const int64_t max_n = on == 2 ? kSmallRows : kSmallAuto;
+  if (on == 0 || P.S != 1 || P.n < 1 || P.n > max_n || P.K > kGroup || req.want_ranges)
+    return false;
+  if ((uint64_t)P.n >= (uint64_t)config().bf_threshold.load()) return false;
+  const SysDesc& d = P.desc[0];
+  if (!d.has_cell || on == 2) return true;
+  double inv[9], w[3];
+  if (!inverse3(d.h, inv)) return false;
+  widths3(inv, w);
+  return d.rc_max * 2.0 < 0.999 * std::fmin(std::fmin(w[0], w[1]), w[2]);
+}
+
+// 0: done, 1: error, 2: the plan took the cell-list path (nothing written)
+int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
+  const int K = P.K;
+  double* dpos = const_cast<double*>(req.pos);
+  if (!P.pos_device) {
+    dpos = (double*)scratch(0, sizeof(double) * 3 * (size_t)P.n);
+    if (!dpos) {
+      set_error("out of device memory");
+      return 1;
+    }
+  }
+  char* small = (char*)scratch(kSmallSlot, 16 * (kGroup + 1));
+  int64_t* hb = (int64_t*)pinned(8 * (kGroup + 1));
+  if (!small || !hb) {
+    set_error(small ? "out of pinned host memory" : "out of device memory");
+    return 1;
+  }
+  SmallArgs a{};
+  a.pos = dpos;
+  a.desc = P.desc[0];
+  for (int q = 0; q < K; ++q) a.rc2[q] = P.rc2[q];
+  a.prm = P.prm;
+  a.bounds = (int64_t*)small;
+  a.band = (float2*)(small + 8 * (kGroup + 1));
+  a.k = K;
+  a.disjoint = P.disjoint;
+  a.full = P.full;
+  hipStream_t st = stream_;
+  if (req.has_ustream) {
+    NL_CHECK(hipEventRecord(ev_[5], req.ustream));
+    NL_CHECK(hipStreamWaitEvent(st, ev_[5], 0));
+  }
+  if (!P.pos_device)
+    NL_CHECK(hipMemcpyAsync(dpos, req.pos, sizeof(double) * 3 * P.n, hipMemcpyHostToDevice, st));
+  // room from the last request of the same shape, else a small default;
+  // a larger total is counted and the launch repeated with exact room
+  const int64_t sig[4] = {P.n, 1, K, (int64_t)req.full * 4 + (int64_t)req.multi * 2 + req.disjoint};
+  const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
+  const int64_t cap = (have_est ? last_total_ + last_total_ / 8 : 0) + 4096;
+  auto launch = [&](int64_t room) -> int {
+    if (alloc_outputs(res, room, req.want_dist, req.want_disp)) return 1;
+    a.edges = (int64_t*)res->edges;
+    a.shifts = (int32_t*)res->shifts;
+    a.dist = (double*)res->dist;
+    a.disp = (double*)res->disp;
+    a.capacity = room;
+    if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
+    NL_CHECK(launch_small(a, st));
+    if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
+    NL_CHECK(hipMemcpyAsync(hb, a.bounds, 8 * (K + 1), hipMemcpyDeviceToHost, st));
+    NL_CHECK(hipStreamSynchronize(st));
+    return 0;
+  };
+  if (launch(cap)) return 1;
+  if (hb[K] < 0) return 2;
+  const int64_t total = hb[K];
+  if (total > cap && launch(total)) return 1;
+  res->total = total;
+  for (int q = 0; q < K; ++q) {
+    const int r = P.rank_of(q);
+    res->base[q] = hb[r];
+    res->len[q] = hb[r + 1] - hb[r];
+  }
+  for (int q = 0; q < 4; ++q) last_sig_[q] = sig[q];
+  last_total_ = total;
+  if (timing_) {
+    float tm = 0;
+    (void)hipEventElapsedTime(&tm, ev_[0], ev_[4]);
+    times_ = StageTimes{};
+    times_.count = times_.total = tm;
+    times_.units = P.n;
+    times_.edges = total;
+  }
   return 0;
 }
 

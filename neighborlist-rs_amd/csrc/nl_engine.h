@@ -26,6 +26,9 @@ struct Config {
   std::atomic<int> group{4};     // home atoms per cell-list work item
   std::atomic<int> parts{1};     // half-stencil slices per home group
   std::atomic<int> autotune{1};  // pick the four above per workload class
+  // single small brute-force systems in one launch (k_small): 0 off, 1 on,
+  // 2 on even where the host expects the cell-list path (the plan decides)
+  std::atomic<int> small_path{1};
 };
 Config& config();
 
@@ -208,6 +211,8 @@ class Engine {
 
  private:
   int run_with(const Request& req, const Tuning& t, Result** out);
+  bool small_eligible(const Request& req, const Pipeline& P) const;
+  int run_small(const Request& req, const Pipeline& P, Result* res);
   void* scratch(int slot, size_t bytes);
   void* pinned(size_t bytes);
 

@@ -39,6 +39,25 @@ struct PairArgs {
   int32_t parts;             // slices of the half stencil per home group
 };
 
+// One single system of at most kSmallRows atoms for k_small: its descriptor
+// before planning (atom_start 0), ascending squared cutoffs and the outputs.
+struct SmallArgs {
+  const double* pos;    // raw positions (n, 3), f64
+  SysDesc desc;
+  double rc2[kGroup];   // k squared cutoffs, ascending
+  Params prm;
+  float2* band;         // [k], written by the plan (cell-list path only)
+  int64_t* bounds;      // [k + 1]: first edge of each list, total; bounds[k] = -1
+                        // when the plan takes the cell-list path (nothing written)
+  int64_t* edges;       // outputs laid out as for the pair kernels
+  int32_t* shifts;
+  double* dist;         // optional: |r_ij|
+  double* disp;         // optional: r_ij
+  int64_t capacity;     // edges the outputs hold; a larger total writes only bounds
+  int32_t k, disjoint, full;
+};
+
+hipError_t launch_small(const SmallArgs& a, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st);
 hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity,

@@ -133,16 +133,17 @@ __global__ void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restri
 // cell-list distance uses wrapped positions and the image shift only
 // (search.rs:540-549, 695-697), so any grid with a covering stencil gives
 // the reference's edge set.
-__global__ __launch_bounds__(1024) void k_plan(const double* __restrict__ pos,
-                                              SysDesc* __restrict__ sys,
-                                              float2* __restrict__ band,
-                                              const double* __restrict__ rc2, Params p) {
-  const int s = blockIdx.x;
-  SysDesc* d = sys + s;
+//
+// plan_system: the plan of system `d`, called by every thread of a block of
+// 1024 (k_plan, and k_small for a single small system); `band` and `rc2s`
+// are this system's [k] thresholds and squared cutoffs.  Thread 0 writes the
+// descriptor; the caller synchronises the block before reading it.
+__device__ __forceinline__ void plan_system(const double* __restrict__ pos, SysDesc* d,
+                                            float2* __restrict__ band,
+                                            const double* __restrict__ rc2s, const Params& p) {
   const int64_t a0 = d->atom_start;
   const int n = d->n_atoms;
   const double rc_max = d->rc_max;
-  const double* rc2s = p.per_sys_rc ? rc2 + (int64_t)s * p.k : rc2;
   // bounding box of an isolated system: 16 waves, four atoms per thread in
   // flight (a system of 50,000 atoms is 13 rounds of loads, not 200); the
   // fold skips NaN and does not depend on the order
@@ -298,8 +299,17 @@ __global__ __launch_bounds__(1024) void k_plan(const double* __restrict__ pos,
     double hi = (rc + delta) * (rc + delta);
     // round the thresholds outward so the certain-accept / certain-reject
     // decisions stay conservative
-    band[(int64_t)s * p.k + q] = make_float2(__double2float_rd(lo), __double2float_ru(hi));
+    band[q] = make_float2(__double2float_rd(lo), __double2float_ru(hi));
   }
+}
+
+__global__ __launch_bounds__(1024) void k_plan(const double* __restrict__ pos,
+                                              SysDesc* __restrict__ sys,
+                                              float2* __restrict__ band,
+                                              const double* __restrict__ rc2, Params p) {
+  const int s = blockIdx.x;
+  plan_system(pos, sys + s, band + (int64_t)s * p.k, p.per_sys_rc ? rc2 + (int64_t)s * p.k : rc2,
+              p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1400,6 +1410,137 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 }
 
 // ---------------------------------------------------------------------------
+// One small system in one launch.  A single system of at most kSmallRows
+// atoms on the brute-force path (single.rs:60-89) is planned, counted,
+// scanned and filled by one block of 16 waves.  The pipeline spends nine
+// dependent launches on such a system, mostly launch and dependency latency
+// around a few microseconds of work.  The plan is plan_system and the rows
+// are unit_bruteforce, in the same row order with the same per-row offsets,
+// so the lists are byte-identical to the pipeline's.  When the plan takes
+// the cell-list path, bounds[k] = -1 and nothing else is written.
+template <int NQ, bool FULL, bool DISP>
+__global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
+  constexpr int kPer = kGroup * kSmallRows / 1024;  // scan entries per thread
+  __shared__ SysDesc s_d;
+  __shared__ double s_rc2[kGroup];
+  __shared__ double s_pos[3 * kSmallRows];         // the system's positions
+  __shared__ uint32_t s_off[kGroup * kSmallRows];  // [list][row]: count, then offset
+  __shared__ uint32_t s_wsum[16];
+  __shared__ int64_t s_bounds[kGroup + 1];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_d = a.desc;
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) s_rc2[g] = a.rc2[g];
+  }
+  __syncthreads();
+  plan_system(a.pos, &s_d, a.band, s_rc2, a.prm);
+  __syncthreads();
+  const int mode = s_d.mode;
+  if (mode == kCellList) {
+    if (tid == 0) a.bounds[a.k] = -1;
+    return;
+  }
+  const int n = s_d.n_atoms;
+  const int k = a.k;
+  // the rows read every position many times: from LDS, not through the caches
+  for (int e = tid; e < 3 * n; e += 1024) s_pos[e] = a.pos[e];
+  __syncthreads();
+  PairArgs A{};
+  A.pos = s_pos;
+  A.global_index = a.prm.global_index;
+  A.edges = a.edges;
+  A.shifts = a.shifts;
+  A.dist = a.dist;
+  A.disp = a.disp;
+  Cuts<NQ> C;
+  C.cnt = k;
+  C.disjoint = a.disjoint != 0;
+  C.prev = -INFINITY;  // every list in one launch (k <= kGroup)
+  C.prev_lo = C.prev_hi = -INFINITY;
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    C.r[g] = g < k ? s_rc2[g] : -INFINITY;
+    C.lo[g] = C.hi[g] = -INFINITY;
+    C.base[g] = C.len[g] = 0;
+  }
+  const int w = uniform_i32(tid >> 6);
+  const int lane = lane_id();
+  // count pass: row i by wave i % 16
+  for (int i = w; i < n; i += 16) {
+    Acc<false> cur[NQ];
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) cur[g] = 0;
+    unit_bruteforce<false, NQ, FULL, DISP>(A, C, &s_d, (int64_t)i, mode, cur);
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) {
+      uint32_t t = cur[g];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      if (FULL) t *= 2u;
+      if (g < k && lane == 0) s_off[g * n + i] = t;
+    }
+  }
+  __syncthreads();
+  // exclusive scan over [list][row]: offsets over all lists, as the
+  // pipeline's scan of [list][item] counts (items off the path count zero)
+  const int len = k * n;
+  const int t0 = tid * kPer;
+  uint32_t v[kPer], sum = 0;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    v[e] = t0 + e < len ? s_off[t0 + e] : 0u;
+    sum += v[e];
+  }
+  uint32_t incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t run = 0, total = 0;
+  for (int x = 0; x < 16; ++x) {
+    const uint32_t y = s_wsum[x];
+    run += x < w ? y : 0u;
+    total += y;
+  }
+  run += incl - sum;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    if (t0 + e < len) s_off[t0 + e] = run;
+    run += v[e];
+  }
+  __syncthreads();
+  if (tid <= k) {
+    const int64_t b = tid < k ? (int64_t)s_off[tid * n] : (int64_t)total;
+    s_bounds[tid] = b;
+    a.bounds[tid] = b;
+  }
+  __syncthreads();
+  if ((int64_t)total > a.capacity) return;  // the caller re-runs with room
+  // fill pass
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    C.base[g] = g < k ? s_bounds[g] : 0;
+    C.len[g] = g < k ? s_bounds[g + 1] - s_bounds[g] : 0;
+  }
+  for (int i = w; i < n; i += 16) {
+    Acc<true> cur[NQ];
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) cur[g] = g < k ? (int64_t)s_off[g * n + i] : 0;
+    unit_bruteforce<true, NQ, FULL, DISP>(A, C, &s_d, (int64_t)i, mode, cur);
+  }
+  if (a.dist) {
+    // the fill stored squared distances (as the pipeline's fill pass); the
+    // block's stores are visible to it after the barrier
+    __syncthreads();
+    for (int64_t e = tid; e < (int64_t)total; e += 1024) a.dist[e] = sqrt(a.dist[e]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan of u32 counts into int64 offsets, decoupled look-back.
 // bounds[q] = offset of element q*seg (q < nseg), bounds[nseg] = total.
 __global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
@@ -1747,6 +1888,36 @@ hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const S
   if (n_items <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_items, dim3(grid_for(n_items, 1024)), dim3(1024), 0, st, keys, cell_start,
                      sys, item_base, n_sys, n_items, group, parts, list, count);
+  return hipGetLastError();
+}
+
+template <int NQ>
+static void launch_small_nq(const SmallArgs& a, hipStream_t st) {
+  const bool disp = a.disp != nullptr;
+  if (a.full) {
+    if (disp)
+      hipLaunchKernelGGL((k_small<NQ, true, true>), dim3(1), dim3(1024), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_small<NQ, true, false>), dim3(1), dim3(1024), 0, st, a);
+  } else {
+    if (disp)
+      hipLaunchKernelGGL((k_small<NQ, false, true>), dim3(1), dim3(1024), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_small<NQ, false, false>), dim3(1), dim3(1024), 0, st, a);
+  }
+}
+
+hipError_t launch_small(const SmallArgs& a, hipStream_t st) {
+  // one block of 1024 threads: the plan's reduction and the scan assume it
+  if (a.k < 1 || a.k > kGroup || a.desc.n_atoms < 1 || a.desc.n_atoms > kSmallRows ||
+      a.desc.atom_start != 0)
+    return hipErrorInvalidValue;
+  if (a.k == 1)
+    launch_small_nq<1>(a, st);
+  else if (a.k <= 4)
+    launch_small_nq<4>(a, st);
+  else
+    launch_small_nq<8>(a, st);
   return hipGetLastError();
 }
 
