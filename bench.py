@@ -624,14 +624,15 @@ def main():
     small_calls = {"workload": "isolated clusters at 0.1 atoms/A^3, rc 6.0, one call each through "
                                "nl_search_batch_device (positions on the device, lists left there); "
                                "median host-clock time per call over 200 calls",
-                   "timing": "one_launch = nl_set_small_path(1) (the default), pipeline = "
+                   "timing": "small_path_us = nl_set_small_path(1), the default (one launch "
+                             "up to 128 atoms, two launches above), pipeline_us = "
                              "nl_set_small_path(0)"}
-    for n_s in (9, 100):
+    for n_s in (9, 100, 500):
         ps = np.random.default_rng(n_s).uniform(0.0, (n_s / 0.1) ** (1 / 3), (n_s, 3))
         dps = DevArray(lib, ps)
         ss = np.array([n_s], np.int64)
         row = {}
-        for mode, key in ((1, "one_launch_us"), (0, "pipeline_us")):
+        for mode, key in ((1, "small_path_us"), (0, "pipeline_us")):
             lib.nl_set_small_path(mode)
             run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)), np.zeros(1, np.uint8),
                         np.zeros(3, np.uint8), np.array([6.0]), False)

@@ -93,15 +93,17 @@ int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* tota
  * path (0/1).  Neither changes results. */
 void nl_set_cell_div(int d);
 int nl_get_cell_div(void);
-/* Single small systems below the brute-force threshold run as one kernel
- * launch (plan, counts, scan and fill in one block) instead of the
- * nine-launch pipeline; the lists are byte-identical either way.
- * 1 (the default): on for systems of at most 128 atoms, isolated or in cells
- * whose widths leave twice the largest cutoff clear of the minimum-image
- * bound (where one block beats the pipeline's spread over the chip);
- * 2: on for every single system of at most 1024 atoms (the kernel's plan
- * hands cell-list systems back to the pipeline); 0: off.  No counterpart in
- * the reference (a GPU launch choice). */
+/* Single systems of at most 1024 atoms below the brute-force threshold run
+ * without the nine-launch pipeline: up to 128 atoms (74 with the
+ * minimum-image branch) as one kernel launch (plan, counts, scan and fill in
+ * one block), above that as two (counts over many blocks with the scan in
+ * the last block, then the fill).  The lists
+ * are byte-identical either way.  1 (the default): on, for isolated systems
+ * and for cells whose widths leave twice the largest cutoff clear of the
+ * minimum-image bound; 2: one launch, 3: two launches, for every single
+ * system of at most 1024 atoms (the kernels' plan hands cell-list systems
+ * back to the pipeline); 0: off.  No counterpart in the reference (a GPU
+ * launch choice). */
 void nl_set_small_path(int mode);
 int nl_get_small_path(void);
 void nl_set_fp32_filter(int on);

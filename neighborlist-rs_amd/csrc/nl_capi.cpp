@@ -530,7 +530,7 @@ int nl_last_timing(float* build_ms, float* count_ms, float* fill_ms, float* tota
 void nl_set_cell_div(int d) { nl::config().cell_div.store(d < 0 ? 0 : (d > 4 ? 4 : d)); }
 int nl_get_cell_div(void) { return nl::config().cell_div.load(); }
 void nl_set_small_path(int mode) {
-  nl::config().small_path.store(mode < 0 ? 0 : (mode > 2 ? 2 : mode));
+  nl::config().small_path.store(mode < 0 ? 0 : (mode > 3 ? 3 : mode));
 }
 int nl_get_small_path(void) { return nl::config().small_path.load(); }
 void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }

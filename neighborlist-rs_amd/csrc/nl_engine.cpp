@@ -364,7 +364,7 @@ hipError_t Pipeline::enqueue_bounds(const Buffers& b, char* pin, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Engine
 
-enum { kRangesSlot = kNumBuffers, kSmallSlot, kSlots };
+enum { kRangesSlot = kNumBuffers, kSlots };
 
 Engine::Engine(int device) : device_(device), scratch_(kSlots) {
   (void)hipSetDevice(device_);
@@ -632,35 +632,55 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
 }
 
 // ---------------------------------------------------------------------------
-// Single small systems: one launch of k_small instead of the pipeline.  The
-// plan inside the kernel is the pipeline's, so the host only screens: one
-// system below the brute-force threshold, at most kGroup lists and no
-// per-system ranges; by default at most kSmallAuto atoms and, with a cell,
-// widths that leave the cutoff clear of the minimum-image bound (otherwise
-// the pipeline runs at once rather than after a declined launch).  One block
-// does all rows, so the pipeline's spread over the chip wins beyond about
-// 150-300 atoms (measured: 33 vs 80 us at 9 atoms, 46 vs 80 us at 100,
-// 74 vs 79 us at 200 with the minimum-image branch); mode 2 lifts both
-// screens up to the kernel's kSmallRows.
+// Single small systems below the brute-force threshold, without the
+// pipeline: one launch of k_small (one block does every row) up to
+// kSmallAuto atoms, two launches (count over many blocks with the scan in the
+// last one, then fill) up to kSmallRows.  The plan inside the kernels is the
+// pipeline's, so the host only screens: one system, at most kGroup lists, no
+// per-system ranges and, with a cell, widths that leave the cutoff clear of
+// the minimum-image bound (otherwise the pipeline runs at once rather than
+// after a declined launch).  Measured per call against the pipeline's
+// 78-95 us (rc 6, device positions): one block 31 us at 9 atoms and 36 us at
+// 100; two launches 40 us at 9 atoms, 44-47 at 100-200, 52-62 at 500, 64-81
+// at 999.  One block wins while its pair work is small: up to 128 atoms on
+// raw positions, about 74 with the minimum-image branch (about three times
+// the work per pair; taken with a cell and under the multi-cutoff rules).
+// Modes 2 and 3 force one form for every single system up to kSmallRows
+// atoms, skipping the host's screen (tests).
 constexpr int64_t kSmallAuto = 128;
 
 bool Engine::small_eligible(const Request& req, const Pipeline& P) const {
   const int on = config().small_path.load();
-  const int64_t max_n = on == 2 ? kSmallRows : kSmallAuto;
-  if (on == 0 || P.S != 1 || P.n < 1 || P.n > max_n || P.K > kGroup || req.want_ranges)
+  if (on == 0 || P.S != 1 || P.n < 1 || P.n > kSmallRows || P.K > kGroup || req.want_ranges)
     return false;
   if ((uint64_t)P.n >= (uint64_t)config().bf_threshold.load()) return false;
   const SysDesc& d = P.desc[0];
-  if (!d.has_cell || on == 2) return true;
+  if (!d.has_cell || on >= 2) return true;
   double inv[9], w[3];
   if (!inverse3(d.h, inv)) return false;
   widths3(inv, w);
   return d.rc_max * 2.0 < 0.999 * std::fmin(std::fmin(w[0], w[1]), w[2]);
 }
 
+// Device block of the small paths (allocated and zeroed once): bounds,
+// band, the count launch's ticket, the planned descriptor, counts, offsets.
+enum : size_t {
+  kSmBounds = 0,
+  kSmBand = 128,
+  kSmTicket = 192,
+  kSmPlanned = 256,
+  kSmCounts = 768,
+  kSmOffs = kSmCounts + 4 * (size_t)kGroup * kSmallRows,
+  kSmBytes = kSmOffs + 4 * (size_t)kGroup * kSmallRows,
+};
+static_assert(sizeof(SysDesc) <= kSmCounts - kSmPlanned, "descriptor slot");
+
 // 0: done, 1: error, 2: the plan took the cell-list path (nothing written)
 int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
   const int K = P.K;
+  const int on = config().small_path.load();
+  const int64_t work = P.n * P.n * ((req.multi || P.desc[0].has_cell) ? 3 : 1);
+  const bool one_block = on == 2 || (on != 3 && work <= kSmallAuto * kSmallAuto);
   double* dpos = const_cast<double*>(req.pos);
   if (!P.pos_device) {
     dpos = (double*)scratch(0, sizeof(double) * 3 * (size_t)P.n);
@@ -669,19 +689,31 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
       return 1;
     }
   }
-  char* small = (char*)scratch(kSmallSlot, 16 * (kGroup + 1));
+  if (!small_buf_) {
+    if (hipMalloc(&small_buf_, kSmBytes) != hipSuccess) {
+      small_buf_ = nullptr;
+      set_error("out of device memory");
+      return 1;
+    }
+    NL_CHECK(hipMemset(small_buf_, 0, kSmBytes));  // the ticket starts at zero
+  }
   int64_t* hb = (int64_t*)pinned(8 * (kGroup + 1));
-  if (!small || !hb) {
-    set_error(small ? "out of pinned host memory" : "out of device memory");
+  if (!hb) {
+    set_error("out of pinned host memory");
     return 1;
   }
+  char* sb = (char*)small_buf_;
   SmallArgs a{};
   a.pos = dpos;
   a.desc = P.desc[0];
   for (int q = 0; q < K; ++q) a.rc2[q] = P.rc2[q];
   a.prm = P.prm;
-  a.bounds = (int64_t*)small;
-  a.band = (float2*)(small + 8 * (kGroup + 1));
+  a.bounds = (int64_t*)(sb + kSmBounds);
+  a.band = (float2*)(sb + kSmBand);
+  a.ticket = (uint32_t*)(sb + kSmTicket);
+  a.planned = (SysDesc*)(sb + kSmPlanned);
+  a.counts = (uint32_t*)(sb + kSmCounts);
+  a.offs = (uint32_t*)(sb + kSmOffs);
   a.k = K;
   a.disjoint = P.disjoint;
   a.full = P.full;
@@ -693,11 +725,11 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
   if (!P.pos_device)
     NL_CHECK(hipMemcpyAsync(dpos, req.pos, sizeof(double) * 3 * P.n, hipMemcpyHostToDevice, st));
   // room from the last request of the same shape, else a small default;
-  // a larger total is counted and the launch repeated with exact room
+  // a larger total is counted and the fill repeated with exact room
   const int64_t sig[4] = {P.n, 1, K, (int64_t)req.full * 4 + (int64_t)req.multi * 2 + req.disjoint};
   const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
   const int64_t cap = (have_est ? last_total_ + last_total_ / 8 : 0) + 4096;
-  auto launch = [&](int64_t room) -> int {
+  auto launch = [&](int64_t room, bool count) -> int {
     if (alloc_outputs(res, room, req.want_dist, req.want_disp)) return 1;
     a.edges = (int64_t*)res->edges;
     a.shifts = (int32_t*)res->shifts;
@@ -705,16 +737,22 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
     a.disp = (double*)res->disp;
     a.capacity = room;
     if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
-    NL_CHECK(launch_small(a, st));
+    if (one_block) {
+      NL_CHECK(launch_small(a, st));
+    } else {
+      if (count) NL_CHECK(launch_brute(false, a, st));
+      NL_CHECK(launch_brute(true, a, st));
+      if (a.dist) NL_CHECK(launch_sqrt_dist(a.dist, a.bounds + K, room, st));
+    }
     if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
     NL_CHECK(hipMemcpyAsync(hb, a.bounds, 8 * (K + 1), hipMemcpyDeviceToHost, st));
     NL_CHECK(hipStreamSynchronize(st));
     return 0;
   };
-  if (launch(cap)) return 1;
+  if (launch(cap, true)) return 1;
   if (hb[K] < 0) return 2;
   const int64_t total = hb[K];
-  if (total > cap && launch(total)) return 1;
+  if (total > cap && launch(total, false)) return 1;
   res->total = total;
   for (int q = 0; q < K; ++q) {
     const int r = P.rank_of(q);

@@ -26,8 +26,9 @@ struct Config {
   std::atomic<int> group{4};     // home atoms per cell-list work item
   std::atomic<int> parts{1};     // half-stencil slices per home group
   std::atomic<int> autotune{1};  // pick the four above per workload class
-  // single small brute-force systems in one launch (k_small): 0 off, 1 on,
-  // 2 on even where the host expects the cell-list path (the plan decides)
+  // single small brute-force systems without the pipeline: 0 off, 1 on (one
+  // block up to 128 atoms, two launches up to 1024), 2 one block and 3 two
+  // launches for every single system up to 1024 atoms (the plan decides)
   std::atomic<int> small_path{1};
 };
 Config& config();
@@ -234,6 +235,7 @@ class Engine {
   bool timing_ = false;
   StageTimes times_;
   void* pinned_ = nullptr;  // host staging for small transfers (hipHostMalloc)
+  void* small_buf_ = nullptr;  // device block of the single-small-system paths
   size_t pinned_cap_ = 0;
   // output-size estimate of the last request shape: lets the fill pass be
   // launched without waiting for the counts

@@ -55,9 +55,17 @@ struct SmallArgs {
   double* disp;         // optional: r_ij
   int64_t capacity;     // edges the outputs hold; a larger total writes only bounds
   int32_t k, disjoint, full;
+  // two-launch form (launch_brute) only
+  uint32_t* counts;     // [k][n] per-row counts
+  uint32_t* offs;       // [k][n] per-row offsets over all lists
+  SysDesc* planned;     // the descriptor after planning (count -> fill)
+  uint32_t* ticket;     // zero before the count launch; the last block re-arms it
 };
 
+// one block (k_small)
 hipError_t launch_small(const SmallArgs& a, hipStream_t st);
+// count launch (fill = false; writes bounds), then fill launch (fill = true)
+hipError_t launch_brute(bool fill, const SmallArgs& a, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st);
 hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity,

@@ -1410,81 +1410,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 }
 
 // ---------------------------------------------------------------------------
-// One small system in one launch.  A single system of at most kSmallRows
-// atoms on the brute-force path (single.rs:60-89) is planned, counted,
-// scanned and filled by one block of 16 waves.  The pipeline spends nine
-// dependent launches on such a system, mostly launch and dependency latency
-// around a few microseconds of work.  The plan is plan_system and the rows
-// are unit_bruteforce, in the same row order with the same per-row offsets,
-// so the lists are byte-identical to the pipeline's.  When the plan takes
-// the cell-list path, bounds[k] = -1 and nothing else is written.
-template <int NQ, bool FULL, bool DISP>
-__global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
-  constexpr int kPer = kGroup * kSmallRows / 1024;  // scan entries per thread
-  __shared__ SysDesc s_d;
-  __shared__ double s_rc2[kGroup];
-  __shared__ double s_pos[3 * kSmallRows];         // the system's positions
-  __shared__ uint32_t s_off[kGroup * kSmallRows];  // [list][row]: count, then offset
-  __shared__ uint32_t s_wsum[16];
-  __shared__ int64_t s_bounds[kGroup + 1];
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    s_d = a.desc;
-#pragma unroll
-    for (int g = 0; g < kGroup; ++g) s_rc2[g] = a.rc2[g];
-  }
-  __syncthreads();
-  plan_system(a.pos, &s_d, a.band, s_rc2, a.prm);
-  __syncthreads();
-  const int mode = s_d.mode;
-  if (mode == kCellList) {
-    if (tid == 0) a.bounds[a.k] = -1;
-    return;
-  }
-  const int n = s_d.n_atoms;
-  const int k = a.k;
-  // the rows read every position many times: from LDS, not through the caches
-  for (int e = tid; e < 3 * n; e += 1024) s_pos[e] = a.pos[e];
-  __syncthreads();
-  PairArgs A{};
-  A.pos = s_pos;
-  A.global_index = a.prm.global_index;
-  A.edges = a.edges;
-  A.shifts = a.shifts;
-  A.dist = a.dist;
-  A.disp = a.disp;
-  Cuts<NQ> C;
-  C.cnt = k;
-  C.disjoint = a.disjoint != 0;
-  C.prev = -INFINITY;  // every list in one launch (k <= kGroup)
-  C.prev_lo = C.prev_hi = -INFINITY;
-#pragma unroll
-  for (int g = 0; g < NQ; ++g) {
-    C.r[g] = g < k ? s_rc2[g] : -INFINITY;
-    C.lo[g] = C.hi[g] = -INFINITY;
-    C.base[g] = C.len[g] = 0;
-  }
-  const int w = uniform_i32(tid >> 6);
-  const int lane = lane_id();
-  // count pass: row i by wave i % 16
-  for (int i = w; i < n; i += 16) {
-    Acc<false> cur[NQ];
-#pragma unroll
-    for (int g = 0; g < NQ; ++g) cur[g] = 0;
-    unit_bruteforce<false, NQ, FULL, DISP>(A, C, &s_d, (int64_t)i, mode, cur);
-#pragma unroll
-    for (int g = 0; g < NQ; ++g) {
-      uint32_t t = cur[g];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-      if (FULL) t *= 2u;
-      if (g < k && lane == 0) s_off[g * n + i] = t;
-    }
-  }
-  __syncthreads();
-  // exclusive scan over [list][row]: offsets over all lists, as the
-  // pipeline's scan of [list][item] counts (items off the path count zero)
-  const int len = k * n;
+// Single small systems on the brute-force path (single.rs:60-89), without
+// the pipeline's nine dependent launches, which cost such a system ~80 us of
+// launch and dependency latency around a few microseconds of work:
+//   k_small                 one block of 16 waves does everything (best up to
+//                           ~130 atoms)
+//   k_brute_count + _fill   one row per wave over ceil(n / 16) blocks; the
+//                           last count block to finish scans (larger systems)
+// All of them plan with plan_system and walk rows with unit_bruteforce in
+// the pipeline's row order at the pipeline's offsets, so the lists are
+// byte-identical to the pipeline's.  When the plan takes the cell-list path,
+// bounds[k] = -1 and nothing else is written.
+
+// In-place exclusive scan of len <= kGroup * kSmallRows u32 entries in LDS
+// ([list][row]: offsets over all lists, as the pipeline's scan of
+// [list][item] counts, where items off the path count zero) by a block of
+// 1024 threads; returns the total.  Ends with the block synchronised.
+__device__ __forceinline__ uint32_t small_scan(uint32_t* s_off, int len, uint32_t* s_wsum) {
+  constexpr int kPer = kGroup * kSmallRows / 1024;  // entries per thread
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
   const int t0 = tid * kPer;
   uint32_t v[kPer], sum = 0;
 #pragma unroll
@@ -1513,6 +1457,92 @@ __global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
     run += v[e];
   }
   __syncthreads();
+  return total;
+}
+
+// The lists of k_small, k_brute_count and k_brute_fill in registers (all
+// of them in one launch: k <= kGroup).
+template <int NQ>
+__device__ __forceinline__ Cuts<NQ> small_cuts(const double* rc2, int k, int disjoint) {
+  Cuts<NQ> C;
+  C.cnt = k;
+  C.disjoint = disjoint != 0;
+  C.prev = -INFINITY;
+  C.prev_lo = C.prev_hi = -INFINITY;
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    C.r[g] = g < k ? rc2[g] : -INFINITY;
+    C.lo[g] = C.hi[g] = -INFINITY;
+    C.base[g] = C.len[g] = 0;
+  }
+  return C;
+}
+
+__device__ __forceinline__ PairArgs small_pair_args(const SmallArgs& a, const double* pos) {
+  PairArgs A{};
+  A.pos = pos;
+  A.global_index = a.prm.global_index;
+  A.edges = a.edges;
+  A.shifts = a.shifts;
+  A.dist = a.dist;
+  A.disp = a.disp;
+  return A;
+}
+
+// Count pass of row i by one wave: per-list counts, written by lane 0.
+template <int NQ, bool FULL>
+__device__ __forceinline__ void small_count_row(const PairArgs& A, const Cuts<NQ>& C,
+                                                const SysDesc* d, int i, int mode, int k,
+                                                int n, uint32_t* out) {
+  Acc<false> cur[NQ];
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) cur[g] = 0;
+  unit_bruteforce<false, NQ, FULL, false>(A, C, d, (int64_t)i, mode, cur);
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    uint32_t t = cur[g];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (FULL) t *= 2u;
+    if (g < k && lane_id() == 0) out[g * n + i] = t;
+  }
+}
+
+// One block: plan, per-row counts, scan, fill.
+template <int NQ, bool FULL, bool DISP>
+__global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
+  __shared__ SysDesc s_d;
+  __shared__ double s_rc2[kGroup];
+  __shared__ double s_pos[3 * kSmallRows];         // the system's positions
+  __shared__ uint32_t s_off[kGroup * kSmallRows];  // [list][row]: count, then offset
+  __shared__ uint32_t s_wsum[16];
+  __shared__ int64_t s_bounds[kGroup + 1];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_d = a.desc;
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) s_rc2[g] = a.rc2[g];
+  }
+  __syncthreads();
+  plan_system(a.pos, &s_d, a.band, s_rc2, a.prm);
+  __syncthreads();
+  const int mode = s_d.mode;
+  if (mode == kCellList) {
+    if (tid == 0) a.bounds[a.k] = -1;
+    return;
+  }
+  const int n = s_d.n_atoms;
+  const int k = a.k;
+  // the rows read every position many times: from LDS, not through the caches
+  for (int e = tid; e < 3 * n; e += 1024) s_pos[e] = a.pos[e];
+  __syncthreads();
+  const PairArgs A = small_pair_args(a, s_pos);
+  Cuts<NQ> C = small_cuts<NQ>(s_rc2, k, a.disjoint);
+  const int w = uniform_i32(tid >> 6);
+  // count pass: row i by wave i % 16
+  for (int i = w; i < n; i += 16) small_count_row<NQ, FULL>(A, C, &s_d, i, mode, k, n, s_off);
+  __syncthreads();
+  const uint32_t total = small_scan(s_off, k * n, s_wsum);
   if (tid <= k) {
     const int64_t b = tid < k ? (int64_t)s_off[tid * n] : (int64_t)total;
     s_bounds[tid] = b;
@@ -1538,6 +1568,89 @@ __global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
     __syncthreads();
     for (int64_t e = tid; e < (int64_t)total; e += 1024) a.dist[e] = sqrt(a.dist[e]);
   }
+}
+
+// Count launch: row 16 b + w by wave w of block b.  Every block plans the
+// system itself (the same routine on the same inputs: the same plan) and
+// stages the positions in LDS; the last block to finish scans all counts,
+// publishes offsets, bounds and the planned descriptor, and re-arms the
+// ticket for the next call.
+template <int NQ, bool FULL>
+__global__ __launch_bounds__(1024) void k_brute_count(SmallArgs a) {
+  __shared__ SysDesc s_d;
+  __shared__ double s_rc2[kGroup];
+  __shared__ double s_pos[3 * kSmallRows];
+  __shared__ uint32_t s_off[kGroup * kSmallRows];
+  __shared__ uint32_t s_wsum[16];
+  __shared__ uint32_t s_last;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_d = a.desc;
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) s_rc2[g] = a.rc2[g];
+  }
+  __syncthreads();
+  plan_system(a.pos, &s_d, a.band, s_rc2, a.prm);
+  __syncthreads();
+  const int mode = s_d.mode;
+  if (mode == kCellList) {
+    if (blockIdx.x == 0 && tid == 0) a.bounds[a.k] = -1;
+    return;
+  }
+  const int n = s_d.n_atoms;
+  const int k = a.k;
+  for (int e = tid; e < 3 * n; e += 1024) s_pos[e] = a.pos[e];
+  __syncthreads();
+  const PairArgs A = small_pair_args(a, s_pos);
+  const Cuts<NQ> C = small_cuts<NQ>(s_rc2, k, a.disjoint);
+  const int i = (int)blockIdx.x * 16 + uniform_i32(tid >> 6);
+  if (i < n) small_count_row<NQ, FULL>(A, C, &s_d, i, mode, k, n, a.counts);
+  // publish the counts, then take a ticket
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) s_last = atomicAdd(a.ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();  // every other block's counts are visible past its ticket
+  const int len = k * n;
+  for (int e = tid; e < len; e += 1024) s_off[e] = __atomic_load_n(a.counts + e, __ATOMIC_RELAXED);
+  __syncthreads();
+  const uint32_t total = small_scan(s_off, len, s_wsum);
+  for (int e = tid; e < len; e += 1024) a.offs[e] = s_off[e];
+  if (tid <= k) a.bounds[tid] = tid < k ? (int64_t)s_off[tid * n] : (int64_t)total;
+  if (tid == 0) {
+    *a.planned = s_d;
+    *a.ticket = 0u;
+  }
+}
+
+// Fill launch: the rows of the count launch at the scanned offsets; nothing
+// when the plan declined or the lists exceed the outputs.
+template <int NQ, bool FULL, bool DISP>
+__global__ __launch_bounds__(1024) void k_brute_fill(SmallArgs a) {
+  __shared__ SysDesc s_d;
+  __shared__ double s_pos[3 * kSmallRows];
+  const int64_t total = a.bounds[a.k];
+  if (total < 0 || total > a.capacity) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_d = *a.planned;
+  __syncthreads();
+  const int n = s_d.n_atoms;
+  const int k = a.k;
+  for (int e = tid; e < 3 * n; e += 1024) s_pos[e] = a.pos[e];
+  __syncthreads();
+  const int i = (int)blockIdx.x * 16 + uniform_i32(tid >> 6);
+  if (i >= n) return;
+  const PairArgs A = small_pair_args(a, s_pos);
+  Cuts<NQ> C = small_cuts<NQ>(a.rc2, k, a.disjoint);
+  Acc<true> cur[NQ];
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    C.base[g] = g < k ? a.bounds[g] : 0;
+    C.len[g] = g < k ? a.bounds[g + 1] - a.bounds[g] : 0;
+    cur[g] = g < k ? (int64_t)a.offs[g * n + i] : 0;
+  }
+  unit_bruteforce<true, NQ, FULL, DISP>(A, C, &s_d, (int64_t)i, s_d.mode, cur);
 }
 
 // ---------------------------------------------------------------------------
@@ -1907,11 +2020,48 @@ static void launch_small_nq(const SmallArgs& a, hipStream_t st) {
   }
 }
 
+template <int NQ>
+static void launch_brute_nq(bool fill, const SmallArgs& a, hipStream_t st) {
+  const dim3 g((unsigned)((a.desc.n_atoms + 15) / 16)), b(1024);
+  const bool disp = a.disp != nullptr;
+  if (!fill) {
+    if (a.full)
+      hipLaunchKernelGGL((k_brute_count<NQ, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_brute_count<NQ, false>), g, b, 0, st, a);
+  } else if (a.full) {
+    if (disp)
+      hipLaunchKernelGGL((k_brute_fill<NQ, true, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_brute_fill<NQ, true, false>), g, b, 0, st, a);
+  } else {
+    if (disp)
+      hipLaunchKernelGGL((k_brute_fill<NQ, false, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_brute_fill<NQ, false, false>), g, b, 0, st, a);
+  }
+}
+
+static bool small_args_ok(const SmallArgs& a) {
+  return a.k >= 1 && a.k <= kGroup && a.desc.n_atoms >= 1 && a.desc.n_atoms <= kSmallRows &&
+         a.desc.atom_start == 0;
+}
+
+hipError_t launch_brute(bool fill, const SmallArgs& a, hipStream_t st) {
+  if (!small_args_ok(a) || !a.counts || !a.offs || !a.planned || !a.ticket)
+    return hipErrorInvalidValue;
+  if (a.k == 1)
+    launch_brute_nq<1>(fill, a, st);
+  else if (a.k <= 4)
+    launch_brute_nq<4>(fill, a, st);
+  else
+    launch_brute_nq<8>(fill, a, st);
+  return hipGetLastError();
+}
+
 hipError_t launch_small(const SmallArgs& a, hipStream_t st) {
   // one block of 1024 threads: the plan's reduction and the scan assume it
-  if (a.k < 1 || a.k > kGroup || a.desc.n_atoms < 1 || a.desc.n_atoms > kSmallRows ||
-      a.desc.atom_start != 0)
-    return hipErrorInvalidValue;
+  if (!small_args_ok(a)) return hipErrorInvalidValue;
   if (a.k == 1)
     launch_small_nq<1>(a, st);
   else if (a.k <= 4)

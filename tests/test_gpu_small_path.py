@@ -1,11 +1,14 @@
-"""Single small systems run as one kernel launch (k_small: plan, per-row
-counts, scan and fill in one block) instead of the nine-launch pipeline.
+"""Single small systems run without the nine-launch pipeline: one launch
+(k_small: plan, per-row counts, scan and fill in one block) or two
+(k_brute_count over many blocks with the scan in the last block, then
+k_brute_fill).
 
 The lists must be byte-identical to the pipeline's (same order, not only the
-same set): every case runs with the one-launch path on (nl_set_small_path 1,
-the default: up to 128 atoms, or 2: up to the kernel's 1024 rows and even
-where the host expects the cell-list path) and off (0), and the arrays are
-compared as they come.  A few cases are also
+same set): every case runs with the small paths on (nl_set_small_path 1, the
+default: one launch up to 128 atoms, two up to 1024; 2: one launch and 3: two
+launches for every single system up to 1024 atoms, even where the host
+expects the cell-list path) and off (0), and the arrays are compared as they
+come.  A few cases are also
 checked against the oracle.  Each case prints its name before it runs."""
 import numpy as np
 import pytest
@@ -97,7 +100,7 @@ def _cell_for(nr, kind, n):
     return nr.PyCell(H.tolist(), pbc), H, pbc
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_small_path_identical_to_pipeline(nr, small_mode, case, mode):
     import torch
@@ -128,10 +131,11 @@ def test_small_path_identical_to_pipeline(nr, small_mode, case, mode):
         _assert_identical(a, b, name + " (numpy)")
 
 
-def test_small_path_against_oracle(nr, oracle, small_mode):
-    small_mode(2)
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_small_path_against_oracle(nr, oracle, small_mode, mode):
+    small_mode(mode)
     for n, cut, kind in [(9, 6.0, None), (700, 6.0, None), (300, 4.0, "triclinic")]:
-        print("[case] small path vs oracle n", n, "kind", kind, flush=True)
+        print("[case] small path vs oracle n", n, "kind", kind, "mode", mode, flush=True)
         pos = _cluster(n, seed=7 * n)
         cell, H, pbc = _cell_for(nr, kind, n)
         if H is not None:
@@ -144,44 +148,49 @@ def test_small_path_against_oracle(nr, oracle, small_mode):
             assert_same_edges(as_tuple(multi[q]), want[q], f"n {n} disjoint list {q}")
 
 
-def test_small_path_declines_cell_list_systems(nr, small_mode):
+@pytest.mark.parametrize("mode", [2, 3])
+def test_small_path_declines_cell_list_systems(nr, small_mode, mode):
     """A periodic cell too small for the minimum image (cutoff beyond half a
-    width): with mode 2 the kernel's plan takes the cell-list path and hands
-    the system back; the pipeline's lists come out."""
+    width): with modes 2 and 3 the kernels' plan takes the cell-list path and
+    hands the system back; the pipeline's lists come out."""
     import torch
     L = 12.0
     H = np.diag([L, L, L])
     pos = np.random.default_rng(5).uniform(0.0, L, (150, 3))
     cell = nr.PyCell(H.tolist(), [True, True, True])
-    print("[case] small path declined, cutoff 7 in a 12 A cube", flush=True)
+    print("[case] small path declined, cutoff 7 in a 12 A cube, mode", mode, flush=True)
     tpos = torch.from_numpy(pos).cuda()
-    a, b = _both(small_mode, 2, lambda: nr.build_neighborlists_torch(cell, tpos, 7.0))
+    a, b = _both(small_mode, mode, lambda: nr.build_neighborlists_torch(cell, tpos, 7.0))
     _assert_identical(a, b, "declined")
     assert a["edge_index"].shape[1] > 0
 
 
-def test_small_path_grows_its_output(nr, small_mode):
+@pytest.mark.parametrize("mode", [2, 3])
+def test_small_path_grows_its_output(nr, small_mode, mode):
     """Calls of one shape reuse the last total as the output estimate; a
-    larger list (same atoms, larger cutoff) overflows it and the launch is
-    repeated with exact room."""
+    larger list (same atoms, larger cutoff) overflows it and the launch (the
+    fill launch alone, in the two-launch form) is repeated with exact room."""
     import torch
     pos = torch.from_numpy(_cluster(800, seed=11)).cuda()
     sizes = []
     for rc in [2.0, 9.0, 3.0, 12.0]:
-        print("[case] small path growth rc", rc, flush=True)
-        a, b = _both(small_mode, 2, lambda: nr.build_neighborlists_torch(None, pos, rc))
+        print("[case] small path growth rc", rc, "mode", mode, flush=True)
+        a, b = _both(small_mode, mode, lambda: nr.build_neighborlists_torch(
+            None, pos, rc, distances=True))
         _assert_identical(a, b, f"rc {rc}")
         sizes.append(a["edge_index"].shape[1])
     assert sizes[1] > sizes[0] + sizes[0] // 8 + 4096
 
 
 def test_small_path_row_limit_and_threshold(nr, small_mode):
-    """1024 atoms (the kernel's row limit, threshold raised, mode 2) take the
-    one-launch path; 1025 atoms and systems at the threshold take the
-    pipeline.  All agree with the pipeline."""
+    """128 atoms take one launch and 129 two by default; 1024 atoms (the
+    kernels' row limit, threshold raised) take either form; 1025 atoms and
+    systems at the threshold take the pipeline.  All agree with the
+    pipeline."""
     import torch
     nr.set_brute_force_threshold(5000)
-    for n, mode in [(128, 1), (129, 1), (1024, 2), (1025, 2)]:
+    for n, mode in [(128, 1), (129, 1), (1024, 1), (1025, 1), (1024, 2), (1024, 3), (1, 3),
+                    (17, 3)]:
         print("[case] small path rows", n, "mode", mode, flush=True)
         pos = torch.from_numpy(_cluster(n, seed=n)).cuda()
         a, b = _both(small_mode, mode, lambda: nr.build_neighborlists_torch(None, pos, 5.0))
