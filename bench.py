@@ -100,6 +100,43 @@ def run_ex(lib, L, dpos, n, sizes, H9, has, pbc, cut, multi, opts):
     return edges
 
 
+def small_single_systems(lib, L, calls=200):
+    """Single small isolated systems, one call each, through the small-system
+    kernels (nl_set_small_path 1, the default) and through the pipeline (0).
+    Engine timing events are off here: per-call latency as a caller sees it."""
+    lib.nl_set_timing(0)
+    small_calls = {"workload": "isolated clusters at 0.1 atoms/A^3, rc 6.0, one call each through "
+                               "nl_search_batch_device (positions on the device, lists left there); "
+                               f"median host-clock time per call over {calls} calls, engine "
+                               "timing events off",
+                   "timing": "small_path_us = nl_set_small_path(1), the default (one launch "
+                             "up to 128 atoms, two launches above), pipeline_us = "
+                             "nl_set_small_path(0)"}
+    for n_s in (9, 100, 500):
+        ps = np.random.default_rng(n_s).uniform(0.0, (n_s / 0.1) ** (1 / 3), (n_s, 3))
+        dps = DevArray(lib, ps)
+        ss = np.array([n_s], np.int64)
+        row = {}
+        for mode, key in ((1, "small_path_us"), (0, "pipeline_us")):
+            lib.nl_set_small_path(mode)
+            run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)), np.zeros(1, np.uint8),
+                        np.zeros(3, np.uint8), np.array([6.0]), False)
+            tt = []
+            for _ in range(calls):
+                t0 = time.perf_counter()
+                es = run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)),
+                                 np.zeros(1, np.uint8), np.zeros(3, np.uint8), np.array([6.0]),
+                                 False)
+                tt.append(time.perf_counter() - t0)
+            row[key] = round(float(np.median(tt)) * 1e6, 1)
+            row["edges"] = es
+        lib.nl_set_small_path(1)
+        dps.free()
+        small_calls[f"{n_s}_atoms"] = row
+    lib.nl_set_timing(1)
+    return small_calls
+
+
 def md_skin_trajectory(lib, L, p3, H3, rc=6.0, skin=1.0, steps=200, sigma=0.01, seed=3):
     """Skin lists on the 10k-atom triclinic system along a random walk
     (sigma per coordinate per step): time per update against a full
@@ -436,6 +473,8 @@ def main():
                     help="'auto' (autotuner) or cell_div,group,parts,fp32 pinned")
     ap.add_argument("--only-serving", action="store_true",
                     help="run only the serving trace (for rocprofv3)")
+    ap.add_argument("--only-small", action="store_true",
+                    help="run only the single-small-system calls (for rocprofv3)")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the timed batch steps (for rocprofv3)")
     ap.add_argument("--serve-requests", type=int, default=600,
@@ -472,6 +511,10 @@ def main():
         lib.nl_device_free(C.c_void_p(a))
         lib.nl_device_free(C.c_void_p(b))
         log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB device-to-device")
+
+    if args.only_small:
+        print(json.dumps({"small_single_systems": small_single_systems(lib, L)}), flush=True)
+        return
 
     if args.only_serving:
         print(json.dumps({"serving": serve_trace(L, args.serve_requests, args.serve_slots,
@@ -620,34 +663,7 @@ def main():
     assert plain_edges == edges, (plain_edges, edges)
     log(f"plain: {plain_sys_per_s:.1f} systems/s vs batched {sys_per_s:.1f}")
 
-    # ---- single small isolated systems: one launch vs the pipeline -------
-    small_calls = {"workload": "isolated clusters at 0.1 atoms/A^3, rc 6.0, one call each through "
-                               "nl_search_batch_device (positions on the device, lists left there); "
-                               "median host-clock time per call over 200 calls",
-                   "timing": "small_path_us = nl_set_small_path(1), the default (one launch "
-                             "up to 128 atoms, two launches above), pipeline_us = "
-                             "nl_set_small_path(0)"}
-    for n_s in (9, 100, 500):
-        ps = np.random.default_rng(n_s).uniform(0.0, (n_s / 0.1) ** (1 / 3), (n_s, 3))
-        dps = DevArray(lib, ps)
-        ss = np.array([n_s], np.int64)
-        row = {}
-        for mode, key in ((1, "small_path_us"), (0, "pipeline_us")):
-            lib.nl_set_small_path(mode)
-            run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)), np.zeros(1, np.uint8),
-                        np.zeros(3, np.uint8), np.array([6.0]), False)
-            tt = []
-            for _ in range(200):
-                t0 = time.perf_counter()
-                es = run_systems(lib, L, dps.ptr, n_s, ss, np.zeros((1, 9)),
-                                 np.zeros(1, np.uint8), np.zeros(3, np.uint8), np.array([6.0]),
-                                 False)
-                tt.append(time.perf_counter() - t0)
-            row[key] = round(float(np.median(tt)) * 1e6, 1)
-            row["edges"] = es
-        lib.nl_set_small_path(1)
-        dps.free()
-        small_calls[f"{n_s}_atoms"] = row
+    small_calls = small_single_systems(lib, L)
     log(f"small single systems: {small_calls}")
 
     # ---- 10k-atom triclinic PBC at rc 6 (single system) ------------------
