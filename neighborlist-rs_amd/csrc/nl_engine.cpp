@@ -697,11 +697,17 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
     }
     NL_CHECK(hipMemset(small_buf_, 0, kSmBytes));  // the ticket starts at zero
   }
-  int64_t* hb = (int64_t*)pinned(8 * (kGroup + 1));
-  if (!hb) {
+  // page-locked block: the bounds read back, then the host positions (at
+  // most 24 KB: one memcpy and an asynchronous upload instead of the
+  // runtime's staged copy of pageable memory)
+  constexpr size_t kPinPos = 8 * (kGroup + 1) + 56;
+  const size_t pos_bytes = sizeof(double) * 3 * (size_t)P.n;
+  char* pin = (char*)pinned(kPinPos + (P.pos_device ? 0 : pos_bytes));
+  if (!pin) {
     set_error("out of pinned host memory");
     return 1;
   }
+  int64_t* hb = (int64_t*)pin;
   char* sb = (char*)small_buf_;
   SmallArgs a{};
   a.pos = dpos;
@@ -722,8 +728,10 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
     NL_CHECK(hipEventRecord(ev_[5], req.ustream));
     NL_CHECK(hipStreamWaitEvent(st, ev_[5], 0));
   }
-  if (!P.pos_device)
-    NL_CHECK(hipMemcpyAsync(dpos, req.pos, sizeof(double) * 3 * P.n, hipMemcpyHostToDevice, st));
+  if (!P.pos_device) {
+    std::memcpy(pin + kPinPos, req.pos, pos_bytes);
+    NL_CHECK(hipMemcpyAsync(dpos, pin + kPinPos, pos_bytes, hipMemcpyHostToDevice, st));
+  }
   // room from the last request of the same shape, else a small default;
   // a larger total is counted and the fill repeated with exact room
   const int64_t sig[4] = {P.n, 1, K, (int64_t)req.full * 4 + (int64_t)req.multi * 2 + req.disjoint};
