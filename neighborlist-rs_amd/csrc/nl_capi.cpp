@@ -301,6 +301,15 @@ int nl_result_copy(const nl_result* r, size_t list, int64_t* edge_i, int64_t* ed
   const int64_t m = R->len[list];
   if (m == 0) return 0;
   const int64_t b = R->base[list];
+  if (R->mirror) {
+    // page-locked copy made with the search (same layout as the device arrays)
+    const int64_t* e = (const int64_t*)R->mirror;
+    const int32_t* sh = (const int32_t*)((const char*)R->mirror + 16 * R->mirror_cap);
+    if (edge_i) std::memcpy(edge_i, e + 2 * b, 8 * m);
+    if (edge_j) std::memcpy(edge_j, e + 2 * b + m, 8 * m);
+    if (shifts) std::memcpy(shifts, sh + 3 * b, 12 * m);
+    return 0;
+  }
   hipStream_t st = R->eng->stream();
   const int64_t* e = (const int64_t*)R->edges;
   hipError_t err = hipSuccess;

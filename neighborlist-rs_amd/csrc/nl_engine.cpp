@@ -50,6 +50,7 @@ void Result::fence_on_consumer() {
 }
 
 Result::~Result() {
+  if (mirror) nl_host_free(mirror);
   if (!eng) return;
   if (edges) eng->dev_release(edges, edges_bytes, fence);
   if (shifts) eng->dev_release(shifts, shifts_bytes, fence);
@@ -736,7 +737,17 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
   // a larger total is counted and the fill repeated with exact room
   const int64_t sig[4] = {P.n, 1, K, (int64_t)req.full * 4 + (int64_t)req.multi * 2 + req.disjoint};
   const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
-  const int64_t cap = (have_est ? last_total_ + last_total_ / 8 : 0) + 4096;
+  // (never more than the brute force can find: n (n - 1) / 2 pairs per
+  // list, one image each, both directions in full lists)
+  const int64_t most = std::max<int64_t>(1, P.n * (P.n - 1) / 2 * K * (req.full ? 2 : 1));
+  const int64_t cap = std::min(most, (have_est ? last_total_ + last_total_ / 8 : 0) + 4096);
+  // host callers read the lists back: a page-locked copy of small lists is
+  // made before the synchronisation, within the same stream.  It is sized
+  // by the last total of the same shape and dropped if the lists outgrow
+  // it.  It pays for lists of a few hundred edges (measured per numpy call:
+  // 9 atoms, 36 edges, 71 -> 43 us); at 2,131 edges (60 KB) it cost 7 us
+  // more than copying into the caller's arrays after the synchronisation.
+  constexpr int64_t kMirrorMax = int64_t(8) << 10;
   auto launch = [&](int64_t room, bool count) -> int {
     if (alloc_outputs(res, room, req.want_dist, req.want_disp)) return 1;
     a.edges = (int64_t*)res->edges;
@@ -753,6 +764,19 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
       if (a.dist) NL_CHECK(launch_sqrt_dist(a.dist, a.bounds + K, room, st));
     }
     if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
+    if (res->mirror) nl_host_free(res->mirror);
+    res->mirror = nullptr;
+    res->mirror_cap = 0;
+    const int64_t want = count && have_est ? std::min(room, last_total_ + 64) : room;
+    if (!P.pos_device && 28 * want <= kMirrorMax) {
+      char* m = (char*)nl_host_alloc(28 * want);
+      if (m) {
+        NL_CHECK(hipMemcpyAsync(m, a.edges, 16 * want, hipMemcpyDeviceToHost, st));
+        NL_CHECK(hipMemcpyAsync(m + 16 * want, a.shifts, 12 * want, hipMemcpyDeviceToHost, st));
+        res->mirror = m;
+        res->mirror_cap = want;
+      }
+    }
     NL_CHECK(hipMemcpyAsync(hb, a.bounds, 8 * (K + 1), hipMemcpyDeviceToHost, st));
     NL_CHECK(hipStreamSynchronize(st));
     return 0;
@@ -761,6 +785,11 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
   if (hb[K] < 0) return 2;
   const int64_t total = hb[K];
   if (total > cap && launch(total, false)) return 1;
+  if (res->mirror && total > res->mirror_cap) {
+    nl_host_free(res->mirror);  // the lists outgrew the copy: read the device
+    res->mirror = nullptr;
+    res->mirror_cap = 0;
+  }
   res->total = total;
   for (int q = 0; q < K; ++q) {
     const int r = P.rank_of(q);

@@ -156,6 +156,13 @@ struct Fence {
 };
 
 // One request's output (the nl_result of the C ABI).
+}  // namespace nl
+extern "C" {
+void* nl_host_alloc(size_t bytes);  // page-locked host pool (nl_capi.cpp)
+void nl_host_free(void* p);
+}
+namespace nl {
+
 struct Result {
   Engine* eng = nullptr;
   int k = 0;
@@ -170,6 +177,12 @@ struct Result {
   size_t dist_bytes = 0;
   void* disp = nullptr;        // double [3 * capacity] (optional)
   size_t disp_bytes = 0;
+  // page-locked copy of edges [2 * mirror_cap] then shifts [3 * mirror_cap],
+  // made before the search's one synchronisation (small single systems
+  // whose positions came from host memory: the host reads the lists without
+  // another device round trip)
+  void* mirror = nullptr;
+  int64_t mirror_cap = 0;
   std::vector<int64_t> ranges; // [s][list(input order)][2] global edge ids (optional)
   std::vector<int32_t> modes;  // per system (optional, with ranges)
   Tuning tune;

@@ -200,3 +200,18 @@ def test_small_path_row_limit_and_threshold(nr, small_mode):
     print("[case] small path at the threshold (cell-list path)", flush=True)
     a, b = _both(small_mode, 1, lambda: nr.build_neighborlists_torch(None, pos, 5.0))
     assert_same_edges(as_tuple(a), as_tuple(b), "n 100 at threshold 100")
+
+
+def test_small_path_host_copy_of_small_lists(nr, small_mode):
+    """Numpy callers: small lists are copied to page-locked memory with the
+    search and read from there.  The copy is sized by the last total of the
+    same shape; lists that outgrow it are read from the device instead.
+    Same atoms, cutoffs up and down, against the pipeline."""
+    pos = _cluster(40, seed=4)
+    for rc in [1.5, 4.0, 2.0, 6.0, 6.0, 1.0]:
+        print("[case] small path host copy rc", rc, flush=True)
+        a, b = _both(small_mode, 1, lambda: nr.build_neighborlists(None, pos, rc))
+        _assert_identical(a, b, f"rc {rc}")
+        multi_a, multi_b = _both(small_mode, 1, lambda: nr.build_neighborlists_multi(
+            None, pos, [rc, 0.5 * rc], disjoint=True))
+        _assert_identical(multi_a, multi_b, f"rc {rc} multi")
