@@ -232,7 +232,10 @@ int nl_server_result_arrays(nl_server* s, int64_t id, const int64_t** edge_i,
  * unchanged) differs from the current cells; *rebuilt says whether it did.
  * nl_skin_check measures without changing anything.  Device list pointers
  * (nl_skin_device, same layout as nl_result_device) stay valid until the
- * next update that rebuilds.  options: NL_LOCAL_INDEX, NL_FULL.  stream:
+ * next update that rebuilds.  options: NL_LOCAL_INDEX, NL_FULL.  disjoint
+ * must be 0: the lists are cumulative (a disjoint shell would lose pairs
+ * that cross a shell boundary between rebuilds), and a nonzero value is
+ * rejected with an error.  stream:
  * the caller's stream, as for nl_search_ex.  Positions must keep one
  * continuous trajectory (no re-wrapping into the cell between updates, or a
  * wrapped atom counts as a large displacement and forces a rebuild). */

@@ -167,10 +167,6 @@ int check_cutoffs(const double* c, size_t k, const char* msg) {
       nl::set_error(msg);
       return 1;
     }
-  if (k > (size_t)nl::kMaxCutoffs) {
-    nl::set_error("at most 64 cutoffs are supported");
-    return 1;
-  }
   return 0;
 }
 
@@ -588,6 +584,15 @@ int nl_skin_create(const double* d_positions, size_t n, const int64_t* sys_sizes
   }
   if (options & ~(unsigned)(NL_DEVICE_POSITIONS | NL_LOCAL_INDEX | NL_FULL)) {
     nl::set_error("skin lists take NL_LOCAL_INDEX and NL_FULL only");
+    return 1;
+  }
+  if (disjoint) {
+    // a list at rc_q + skin_q holds every pair now within rc_q only when it
+    // is cumulative: a disjoint shell misses pairs that were inside the
+    // previous shell at the build and have moved out since
+    nl::set_error("skin lists are cumulative (disjoint shells cannot keep every pair within "
+                  "rc in its shell between rebuilds); split the cumulative lists by distance "
+                  "instead");
     return 1;
   }
   int ndev = 0;

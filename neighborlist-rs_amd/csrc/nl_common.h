@@ -12,7 +12,6 @@
 
 namespace nl {
 
-constexpr int kMaxCutoffs = 64;     // cutoffs per call (the reference has no limit; 64 is ample)
 constexpr int kGroup = 8;           // cutoffs classified per pair-kernel launch
 constexpr int kWaves = 4;           // waves (units) per pair-kernel block
 constexpr int kRadixMaxBits = 12;   // LSD radix digit width limit (4096 digits)

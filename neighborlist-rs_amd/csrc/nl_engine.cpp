@@ -78,8 +78,10 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   S = (int)req.sys.size();
   K = (int)req.cutoffs.size();
   tune = t;
-  if (K <= 0 || K > kMaxCutoffs) {
-    set_error("number of cutoffs must be between 1 and 64");
+  // any number of cutoffs (search.rs:344-361): every per-list array is sized
+  // by K and the pair kernels take the lists kGroup at a time
+  if (K <= 0) {
+    set_error("at least one cutoff is needed");
     return 1;
   }
   pos_device = req.pos_device;

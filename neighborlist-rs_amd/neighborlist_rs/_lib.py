@@ -4,6 +4,7 @@ The shared library is built in-tree by neighborlist-rs_amd/build.py.  There
 is no CPU fallback: if the library is missing, importing this module raises.
 """
 import ctypes as C
+import importlib.util
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -14,6 +15,35 @@ if not os.path.exists(LIB_PATH):
         f"libnlmi355.so not found at {LIB_PATH}: build it with "
         "`python neighborlist-rs_amd/build.py` (the engine has no CPU fallback)")
 
+
+def _torch_runtime():
+    """Binds the process to torch's bundled HIP runtime when torch is
+    installed, without importing torch.
+
+    torch ships its own libamdhip64 / libhsa-runtime64 (sonames
+    libamdhip64.so.7 / libhsa-runtime64.so.1, the ones this library needs)
+    and loads them by file name from its own directory.  Loaded here first
+    and globally, they satisfy this library's dependencies by soname, and a
+    later `import torch` finds the same files already mapped: the process
+    holds one HIP runtime whichever package is imported first.  Without this,
+    loading the library first maps the system runtime and torch then maps a
+    second one, which finds no device."""
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return []
+    if spec is None or not spec.submodule_search_locations:
+        return []
+    tlib = os.path.join(list(spec.submodule_search_locations)[0], "lib")
+    held = []
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        path = os.path.join(tlib, name)
+        if os.path.exists(path):
+            held.append(C.CDLL(path, mode=C.RTLD_GLOBAL))
+    return held
+
+
+_runtime = _torch_runtime()
 lib = C.CDLL(LIB_PATH)
 
 dp = C.POINTER(C.c_double)

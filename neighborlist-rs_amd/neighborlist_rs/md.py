@@ -25,6 +25,14 @@ from .tensors import (NL_FULL, _batch_cells_t, _batch_sizes, _check_cutoffs,
                       _positions_t, _single_cell, _torch)
 
 
+# Skin lists hold every pair within rc_q + skin_q at the last build; a pair
+# that drifts across a shell boundary between rebuilds would sit in the wrong
+# disjoint shell, so only cumulative lists keep the coverage guarantee.
+_DISJOINT_MSG = ("skin lists are cumulative (disjoint shells cannot keep every pair "
+                 "within rc in its shell between rebuilds); split the cumulative lists "
+                 "by distance instead")
+
+
 def _as_list(x, n, what):
     if np.ndim(x) == 0:
         return [float(x)] * n
@@ -44,6 +52,8 @@ class DynamicNeighborManager:
 
     def __init__(self, positions, cutoffs, skins, cell=None, *, batch=None, cells=None,
                  labels=None, disjoint=False, full=False):
+        if disjoint:
+            raise ValueError(_DISJOINT_MSG)
         torch = _torch()
         pos = _positions_t(positions)
         n = pos.shape[0]
@@ -86,7 +96,7 @@ class DynamicNeighborManager:
         self._cache = None
 
     def close(self):
-        if self._ptr:
+        if getattr(self, "_ptr", None):
             lib.nl_skin_free(self._ptr)
             self._ptr = C.c_void_p()
 

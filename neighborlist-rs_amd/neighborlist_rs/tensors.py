@@ -58,9 +58,11 @@ _runtime_checked = False
 def _torch():
     """torch, after checking that the process holds one HIP runtime.
 
-    torch ships its own HIP runtime.  When torch is imported first the engine
-    library binds to it; when the library was loaded first (with the system
-    runtime) a later torch loads a second one, which then finds no device."""
+    torch ships its own HIP runtime, and the engine library binds to it
+    whichever package is imported first (the loader maps torch's runtime
+    before the library).  A second runtime can only appear when something
+    else mapped another copy first; torch then finds no device, so that is
+    reported here rather than as a failed device call."""
     global _runtime_checked
     import torch
     if not _runtime_checked:
@@ -71,8 +73,8 @@ def _torch():
             libs = set()
         if len(libs) > 1:
             raise RuntimeError(
-                "two HIP runtimes are loaded (" + ", ".join(sorted(libs)) + "): import torch "
-                "before neighborlist_rs so that the engine uses torch's runtime")
+                "two HIP runtimes are loaded (" + ", ".join(sorted(libs)) + "): a runtime "
+                "other than torch's was mapped before neighborlist_rs was imported")
         _runtime_checked = True
     return torch
 

@@ -467,7 +467,7 @@ static void atom_traverse(const trav_t *t, int64_t i, int64_t *counts, nlo_list 
   double max_sq = t->rc2[t->k - 1];
   double plim = sqrt(max_sq) + cl->radius;
   double plim2 = plim * plim;
-  int64_t local[64];
+  int64_t local[t->k]; /* one counter per list (any number of cutoffs) */
   for (int q = 0; q < t->k; ++q) local[q] = 0;
   for (int32_t dx = -cl->ns[0]; dx <= cl->ns[0]; ++dx) {
     int32_t nbx, sxb;
@@ -651,11 +651,11 @@ static int system_search(const double *pos, int64_t n, const double *h, const in
   nlo_perpendicular_widths(c.hinv, w);
   double minw = fmin(fmin(w[0], w[1]), w[2]);
   int mic_safe = rmax * 2.0 < minw;
-  int order[64];
+  int order[k];
   sort_cutoffs(cutoffs, k, order);
-  double rc2[64];
+  double rc2[k];
   for (int q = 0; q < k; ++q) rc2[q] = cutoffs[order[q]] * cutoffs[order[q]];
-  nlo_list tmp[64];
+  nlo_list tmp[k];
   if ((size_t)n < g_bf_threshold && mic_safe) {
     if (!multi && !c.pbc[0] && !c.pbc[1] && !c.pbc[2]) {
       bf_isolated(pos, n, rc2[0], idx_offset, &tmp[0]);
@@ -674,7 +674,9 @@ static int system_search(const double *pos, int64_t n, const double *h, const in
 }
 
 static int check_k(int k) {
-  if (k > 64) return set_err("at most 64 cutoffs are supported");
+  /* any number of cutoffs, as search.rs:344-361 (per-list state is sized
+   * by k on the stack) */
+  if (k > 100000) return set_err("too many cutoffs");
   return NLO_OK;
 }
 

@@ -207,3 +207,24 @@ def test_torch_surface_validation(nr):
         tensors._check_cutoffs([1.0, 2.0], ["a"])
     with pytest.raises(ValueError, match="All cutoffs must be positive and finite"):
         tensors._check_cutoffs([1.0, float("nan")], None)
+
+
+def test_skin_lists_reject_disjoint(nr):
+    """Skin lists are cumulative: a disjoint shell built at rc_q + skin_q
+    would lose pairs that drift across a shell boundary between rebuilds, so
+    both the C ABI and the torch manager refuse disjoint=True before any
+    device work."""
+    from neighborlist_rs import _lib
+    lib = _lib.lib
+    sz = np.array([4], np.int64)
+    cut = np.array([2.0, 3.0])
+    sk = np.array([0.5, 0.5])
+    p = C.c_void_p()
+    rc = lib.nl_skin_create(C.c_void_p(1), 4, sz.ctypes.data_as(_lib.i64p), 1, None, None, None,
+                            cut.ctypes.data_as(_lib.dp), sk.ctypes.data_as(_lib.dp), 2, 1, 1, 0,
+                            None, C.byref(p))
+    assert rc != 0 and not p.value
+    assert "cumulative" in _lib.last_error()
+    from neighborlist_rs.md import DynamicNeighborManager
+    with pytest.raises(ValueError, match="cumulative"):
+        DynamicNeighborManager(np.zeros((4, 3)), [2.0, 3.0], 0.5, disjoint=True)

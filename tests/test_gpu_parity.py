@@ -253,6 +253,35 @@ def test_twenty_cutoffs_vs_oracle(nr, oracle, disjoint):
         assert_same_edges(as_tuple(got[k]), want[k], f"batch list {k} rc={cut[k]:.3f}")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("disjoint", [False, True])
+def test_hundred_cutoffs_vs_oracle(nr, oracle, disjoint):
+    """The reference takes any number of cutoffs (search.rs:344-361,
+    python_api.rs:235-260): 100 lists, unsorted with repeats, thirteen
+    pair-kernel launches per pass; single system and a ragged batch."""
+    from neighborlist_rs import workloads as W
+    rng = np.random.default_rng(100)
+    cut = [float(c) for c in rng.uniform(1.5, 7.0, size=96)] + [3.0, 3.0, 6.5, 2.0]
+    p, H, _ = W.c2_ethanol()
+    _print_case("hundred cutoffs single", disjoint)
+    want = oracle.search_single_multi(p, H, [1, 1, 1], cut, disjoint)
+    got = nr.build_neighborlists_multi(nr.PyCell(H.tolist()), p, cut, disjoint=disjoint)
+    assert len(got) == len(cut)
+    total = 0
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"single list {k} rc={cut[k]:.3f}")
+        total += got[k]["edge_index"].shape[1]
+    assert total > 0
+    p, sizes, cells, _ = W.c4_ragged_batch(n_systems=8, nmax=900, seed=101)
+    batch, rows = W.batch_arrays(sizes, cells)
+    _print_case("hundred cutoffs batch", disjoint)
+    ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+    want = oracle.search_batch_multi(p, batch, ocells, cut, disjoint)
+    got = nr.build_neighborlists_batch_multi(p, batch, rows, cut, disjoint=disjoint)
+    for k in range(len(cut)):
+        assert_same_edges(as_tuple(got[k]), want[k], f"batch list {k} rc={cut[k]:.3f}")
+
+
 def test_parallel_flag_does_not_change_results(nr, oracle):
     """parallel=False selects the reference's serial search (search.rs:275-289);
     the edge set is the same."""

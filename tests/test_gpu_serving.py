@@ -10,7 +10,7 @@ import threading
 import numpy as np
 import pytest
 
-from conftest import as_tuple, assert_same_edges
+from conftest import as_tuple, assert_same_edge_sets, assert_same_edges
 
 pytestmark = pytest.mark.gpu
 
@@ -60,6 +60,35 @@ def test_serving_mix_equals_alone_and_oracle(nr, oracle):
         assert info_m[k][0] == m["shift"].shape[0]
         plain = as_tuple(nr.build_neighborlists(_cell(nr, H), p, rc))
         assert_same_edges(as_tuple(m), plain, what + " vs plain path")
+
+
+def test_serving_full_size_trace(nr, oracle):
+    """The serving trace at its stated size: N log-uniform in [16, 50000],
+    rc uniform in [4, 12], a third each isolated / orthorhombic / triclinic.
+    Seven of the sixteen requests exceed 4096 atoms, so their batches take the
+    global radix sort, and they share batches with smaller requests, each
+    with its own cutoff.  Every request: the same arrays (order included)
+    served alone and in the mix, and the oracle's edge set."""
+    from neighborlist_rs.workloads import c5_serving_trace
+    reqs = c5_serving_trace(seed=31, n_requests=16)
+    sizes = [p.shape[0] for (_, p, _, _) in reqs]
+    assert sum(n > 4096 for n in sizes) == 7 and max(sizes) > 30000
+    _print_case("serving full size alone", sizes)
+    alone, _, st_a = _serve(nr, reqs, slots=1, max_systems=1)
+    assert st_a["max_batch"] == 1
+    _print_case("serving full size mix")
+    mix, info, st = _serve(nr, reqs, slots=3)
+    assert st["completed"] == len(reqs)
+    # a batch holding a request over 4096 atoms together with other requests
+    assert any(b > 1 and n > 4096 for n, (_, _, b) in zip(sizes, info)), info
+    for k, ((_, p, H, rc), a, m) in enumerate(zip(reqs, alone, mix)):
+        what = f"request {k}: n={p.shape[0]} rc={rc:.3f} cell={H is not None}"
+        _print_case(what)
+        np.testing.assert_array_equal(a["edge_index"], m["edge_index"], err_msg=what)
+        np.testing.assert_array_equal(a["shift"], m["shift"], err_msg=what)
+        want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
+        assert_same_edge_sets(as_tuple(m), want, what)
+        assert info[k][0] == m["shift"].shape[0]
 
 
 def test_serving_threads_and_budget(nr, oracle):

@@ -57,13 +57,36 @@ def test_perpendicular_widths(oracle):
     assert oracle.perpendicular_widths(H)[0] == pytest.approx(1.0 / SQ2)
 
 
-def test_inverse_is_cofactor_formula(oracle):
-    # the inverse of a diagonal H is (m22 m33)/det etc., not 1/m11: pin the
-    # bits on a case where the two differ
-    H = np.diag([20.462804791791733, 20.462804791791733, 20.462804791791733])
+def test_inverse_identity_and_triclinic_exact(oracle):
+    # cell.rs test_getters: the inverse of the identity IS the identity
+    # (assert_eq!, exact); cell.rs test_triclinic_cell: (13, 10.5, 10) maps to
+    # (1, 1, 1) within approx's default epsilon (f64::EPSILON, relative and
+    # absolute), i.e. to within one ulp of 1
+    assert np.array_equal(oracle.cell_inverse(np.eye(3)), np.eye(3))
+    H = np.array([[10.0, 2.0, 1.0], [0.0, 10.0, 0.5], [0.0, 0.0, 10.0]])
     inv = oracle.cell_inverse(H)
-    m = H[0, 0]
-    assert inv[0, 0] == (m * m) / (m * (m * m))
+    x = np.array([13.0, 10.5, 10.0])
+    frac = [(inv[r, 0] * x[0] + inv[r, 1] * x[1]) + inv[r, 2] * x[2] for r in range(3)]
+    eps = np.finfo(np.float64).eps
+    for f in frac:
+        assert abs(f - 1.0) <= eps, frac
+
+
+def test_inverse_ill_conditioned_cells(oracle):
+    # H^-1 H = I to within a few condition numbers of rounding, on skewed,
+    # nearly singular, non-symmetric cells: a transposed cofactor matrix, a
+    # swapped minor or a wrong determinant sign all leave residuals of O(1)
+    rng = np.random.default_rng(7)
+    eps = np.finfo(np.float64).eps
+    for t in range(200):
+        a = rng.normal(size=(3, 3)) * 10.0
+        a[2] = a[0] + a[1] + rng.normal(size=3) * 10.0 ** rng.uniform(-6, 0)
+        inv = oracle.cell_inverse(a)
+        cond = np.linalg.cond(a)
+        res = np.abs(inv @ a - np.eye(3)).max()
+        assert res <= 64 * cond * eps, (t, res, cond)
+        res2 = np.abs(a @ inv - np.eye(3)).max()
+        assert res2 <= 64 * cond * eps, (t, res2, cond)
 
 
 # ---- search.rs:1413-1910 ---------------------------------------------------
@@ -265,8 +288,15 @@ def test_disjoint_shells(oracle):
     d = [len(r[0]) for r in dis]
     assert n[0] < n[1] < n[2]
     assert d == [n[0], n[1] - n[0], n[2] - n[1]]
-    pairs = [set(zip(*canon(*r)[:2])) for r in dis]
-    assert pairs[0].isdisjoint(pairs[1]) or True  # self-image pairs may repeat across shells
+    # each (i, j, S) edge of the cumulative list lies in exactly one shell
+    trip = [set(zip(*(a.tolist() for a in canon(*r)[:2]), map(tuple, canon(*r)[2].tolist())))
+            for r in dis]
+    for a in range(3):
+        for b in range(a + 1, 3):
+            assert trip[a].isdisjoint(trip[b]), (a, b)
+    full = set(zip(*(a.tolist() for a in canon(*cum[2])[:2]),
+                   map(tuple, canon(*cum[2])[2].tolist())))
+    assert trip[0] | trip[1] | trip[2] == full
     pos2, rows2 = _cu_fcc(2)
     res = oracle.search_single_multi(pos2, rows2, [1, 1, 1], [5.0, 3.0], True)
     small = len(oracle.search_single(pos2, rows2, [1, 1, 1], 3.0)[0])
@@ -278,6 +308,21 @@ def test_twenty_cutoffs(oracle):
     p = np.array([[0.0, 0, 0], [1.0, 0, 0]])
     res = oracle.search_single_multi(p, None, None, [0.1 * i for i in range(1, 21)], False)
     assert len(res) == 20 and len(res[10][0]) == 1 and len(res[8][0]) == 0
+
+
+def test_hundred_cutoffs_match_singles(oracle):
+    # any number of cutoffs (search.rs:344-361): every cumulative list of a
+    # 100-cutoff call is the single-cutoff list at that cutoff, and the
+    # disjoint shells partition the largest one
+    pos, rows = _cu_fcc(2)
+    rng = np.random.default_rng(3)
+    cut = [float(c) for c in rng.uniform(2.0, 6.0, size=97)] + [3.0, 3.0, 5.5]
+    cum = oracle.search_single_multi(pos, rows, [1, 1, 1], cut, False)
+    dis = oracle.search_single_multi(pos, rows, [1, 1, 1], cut, True)
+    assert len(cum) == len(dis) == 100
+    for q in (0, 17, 64, 65, 97, 98, 99):
+        assert_same_edges(cum[q], oracle.search_single(pos, rows, [1, 1, 1], cut[q]), str(q))
+    assert sum(len(r[0]) for r in dis) == len(cum[int(np.argmax(cut))][0])
 
 
 # ---- logged counts for the reference's fixtures ----------------------------
@@ -392,3 +437,21 @@ def test_property_isolated_vs_plain(oracle, n, rc, seed):
             if np.sqrt(((pos[i] - pos[j]) ** 2).sum()) < rc]
     assert sorted(zip(ei.tolist(), ej.tolist())) == sorted(want)
     assert np.all(s == 0)
+
+
+def test_packed_edge_set_comparison_catches_a_changed_edge(oracle):
+    # the fast comparison used for the multi-million-edge serving lists must
+    # see a single changed shift component and a swapped (i, j)
+    from conftest import assert_same_edge_sets
+    pos, rows = _cu_fcc(2)
+    r = oracle.search_single(pos, rows, [1, 1, 1], 6.0)
+    assert_same_edge_sets(r, r, "same")
+    s2 = r[2].copy()
+    s2[7, 2] += 1
+    with pytest.raises(AssertionError):
+        assert_same_edge_sets((r[0], r[1], s2), r, "shift")
+    i2, j2 = r[0].copy(), r[1].copy()
+    k = int(np.nonzero(i2 != j2)[0][0])
+    i2[k], j2[k] = j2[k], i2[k]
+    with pytest.raises(AssertionError):
+        assert_same_edge_sets((i2, j2, r[2]), r, "swap")
