@@ -7,11 +7,16 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <sstream>
 
 namespace nl {
+
+Config::Config() {
+  if (const char* v = std::getenv("NLMI355_XCD_CHUNK")) xcd_chunk = std::max(0, std::atoi(v));
+}
 
 Config& config() {
   static Config c;
@@ -302,6 +307,7 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.full = P.full ? 1 : 0;
   A.group = P.tune.group;
   A.parts = P.tune.parts;
+  A.xcd_chunk = config().xcd_chunk.load();
   return A;
 }
 

@@ -715,7 +715,8 @@ __device__ __forceinline__ void divmod_small(int v, int m, float inv_m, int* q, 
 // order.  Held in registers for the whole wave.  For cumulative lists a
 // pair belongs to list q iff d^2 < rc_q^2; with `disjoint` only to the
 // first list it reaches (search.rs:837-849), i.e. also not to q-1.
-// NQ: lists compiled into the kernel (1, 2, 3, 4 exactly; 8 = up to 8).
+// NQ: lists compiled into the kernel (1, 2, 3, 4 exactly; 8 = up to 8);
+// lists past cnt have -inf thresholds and never take a pair.
 template <int NQ>
 struct Cuts {
   double r[NQ];         // rc^2 of the group's lists
@@ -727,130 +728,185 @@ struct Cuts {
   int64_t len[NQ];      // edges of each list
   int cnt;
   bool disjoint;
-  __device__ bool has(int g) const { return NQ < 8 || g < cnt; }
 };
 
+// The lists of one step as lane masks (bit l: the pair of lane l).  lt[g]:
+// the lanes with d^2 < rc_g^2, nested since the cutoffs ascend.  A
+// cumulative list takes lt[g]; a disjoint shell takes lt[g] without the
+// lanes of the list before it (`below`: those of list q_lo - 1 for g = 0).
+// cm: the lanes whose pair is a candidate at all.  Masks live in scalar
+// registers: the per-list work of a step is scalar, not per lane.
 template <bool DISJ, int NQ>
-__device__ __forceinline__ uint32_t members64(double d2, const Cuts<NQ>& C) {
-  uint32_t m = 0;
+__device__ __forceinline__ void shells(const uint64_t* lt, uint64_t below, uint64_t cm,
+                                       uint64_t* m) {
 #pragma unroll
-  for (int g = 0; g < NQ; ++g)
-    if (C.has(g) && d2 < C.r[g]) m |= 1u << g;
-  if (DISJ) m = (d2 < C.prev) ? 0u : (m & (0u - m));
-  return m;
-}
-template <int NQ>
-__device__ __forceinline__ uint32_t members64_rt(double d2, const Cuts<NQ>& C) {
-  return C.disjoint ? members64<true, NQ>(d2, C) : members64<false, NQ>(d2, C);
+  for (int g = 0; g < NQ; ++g) {
+    uint64_t x = lt[g] & cm;
+    if (DISJ) x &= ~(g > 0 ? lt[g - 1] : below);
+    m[g] = x;
+  }
 }
 
-// FP32 filter: the decision for each threshold is certain when e2 lies
-// below lo (inside) or above hi (outside); `unsure` when any needed
-// threshold is within its band, then the FP64 expression decides.
 template <bool DISJ, int NQ>
-__device__ __forceinline__ uint32_t members32(float e2, const Cuts<NQ>& C, bool* unsure) {
-  uint32_t m = 0;
+__device__ __forceinline__ void lists64(double d2, const Cuts<NQ>& C, uint64_t cm, uint64_t* m) {
+  uint64_t lt[NQ];
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) lt[g] = __ballot(d2 < C.r[g]);
+  const uint64_t below = DISJ ? __ballot(d2 < C.prev) : 0ull;
+  shells<DISJ, NQ>(lt, below, cm, m);
+}
+
+// FP32 filter: per list the lanes certainly inside (e2 < lo); returns the
+// lanes where some threshold lies within its band, whose decisions the
+// FP64 expression takes (recheck64).
+template <bool DISJ, int NQ>
+__device__ __forceinline__ uint64_t filter32(float e2, const Cuts<NQ>& C, uint64_t* lt,
+                                             uint64_t* below) {
   bool u = false;
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
-    if (C.has(g)) {
-      if (e2 < C.lo[g]) m |= 1u << g;
-      u |= (e2 >= C.lo[g]) && (e2 <= C.hi[g]);
-    }
+    const bool a = e2 < C.lo[g];
+    lt[g] = __ballot(a);
+    u |= !a && e2 <= C.hi[g];
   }
   if (DISJ) {
-    u |= (e2 >= C.prev_lo) && (e2 <= C.prev_hi);
-    m = (e2 < C.prev_lo) ? 0u : (m & (0u - m));
+    const bool a = e2 < C.prev_lo;
+    *below = __ballot(a);
+    u |= !a && e2 <= C.prev_hi;
+  } else {
+    *below = 0ull;
   }
-  *unsure = u;
-  return m;
+  return __ballot(u);
 }
 
-// Per-list accumulators: lane-private counts in the count pass (summed
-// over the wave when the work item ends), wave-uniform write cursors in
-// the fill pass.
-template <bool FILL>
-using Acc = typename std::conditional<FILL, int64_t, uint32_t>::type;
+// The lanes of `need` decided again from their FP64 d^2 (+inf elsewhere).
+template <bool DISJ, int NQ>
+__device__ __forceinline__ void recheck64(double d2, const Cuts<NQ>& C, uint64_t need, uint64_t* lt,
+                                          uint64_t* below) {
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) lt[g] = (lt[g] & ~need) | __ballot(d2 < C.r[g]);
+  if (DISJ) *below = (*below & ~need) | __ballot(d2 < C.prev);
+}
 
-// Emit or count one step of up to 64 candidates; `mem` holds the lane's
-// list bits (zero for lanes without a pair).
-//
-// FULL (fill pass of full lists): each pair is written twice, (i, j, S) at
-// an even slot e and (j, i, -S) at e + 1.  Every count is then even, so every
-// list bound and item offset is even and the pair goes out as 16-byte
-// (edge, distance) and 8-byte (shift) stores.  A self image (i, i, S) is
-// emitted once by the half stencil and mirrored to (i, i, -S), which the
-// half list writes as a second emit instead.
-//
-// DISP: (ex, ey, ez), the edge's displacement r_ij in its own orientation,
-// is stored to A.disp (the mirror gets its negation).
-template <bool FILL, int NQ, bool FULL = false, bool DISP = false>
-__device__ __forceinline__ void emit_step(const PairArgs& A, const Cuts<NQ>& C, uint32_t mem, int64_t gi,
-                                 int64_t gj, int sx, int sy, int sz, double d2, double ex,
-                                 double ey, double ez, Acc<FILL>* cur) {
-  if (!FILL) {
+// Count pass: edges per list so far, wave-uniform (scalar registers).
+template <int NQ>
+struct Counts {
+  uint32_t c[NQ];
+};
+
+// Fill pass: per list the item's output, as 64-bit bases at the item's
+// first edge in each array and a 32-bit count of the edges written since.
+// A store takes the base (scalar) plus a 32-bit lane offset (slot x element
+// size): no 64-bit address arithmetic per step.  The bases advance by the
+// count once it passes kRebase (checked once per candidate chunk, which adds
+// at most 64 x 64 x 2 x 2 edges per list), so a slot stays below 2^24 and
+// its byte offsets are 24-bit multiplies (full rate; a 32-bit multiply runs
+// at a quarter of it).
+constexpr uint32_t kRebase = 1u << 23;
+
+template <int NQ, bool DIST, bool DISP>
+struct Out {
+  int64_t* src[NQ];  // sources; targets at src + len of the list
+  int64_t* dst[NQ];
+  int32_t* sh[NQ];   // 3 per edge
+  double* dist[NQ];  // d^2 per edge (DIST)
+  double* disp[NQ];  // r_ij per edge, 3 doubles (DISP)
+  uint32_t c[NQ];
+  __device__ __forceinline__ void init(const PairArgs& A, const Cuts<NQ>& C, const int64_t* e0) {
 #pragma unroll
-    for (int g = 0; g < NQ; ++g) cur[g] += (mem >> g) & 1u;
-    return;
+    for (int g = 0; g < NQ; ++g) {
+      src[g] = A.edges + C.base[g] + e0[g];
+      dst[g] = src[g] + C.len[g];
+      sh[g] = A.shifts + 3 * e0[g];
+      dist[g] = DIST ? A.dist + e0[g] : nullptr;
+      disp[g] = DISP ? A.disp + 3 * e0[g] : nullptr;
+      c[g] = 0;
+    }
   }
+  __device__ __forceinline__ void rebase() {
 #pragma unroll
-  for (int g = 0; g < NQ; ++g) {
-    if (!C.has(g)) break;
-    const bool on = (mem >> g) & 1u;
-    const uint64_t m = __ballot(on);
-    if (m == 0) continue;
-    if (FILL && on) {
-      // plain stores: L2 merges the partial lines of successive steps
-      // (non-temporal stores measured 10% slower here)
-      // wave-uniform bases (the cursor is uniform) plus a 32-bit lane offset
-      const uint32_t lb = lanes_below(m);
-      int64_t* src = A.edges + (C.base[g] + cur[g]);
-      int64_t* dst = src + C.len[g];
-      int32_t* sh = A.shifts + 3 * cur[g];
-      if (FULL) {
-        reinterpret_cast<longlong2*>(src)[lb] = make_longlong2(gi, gj);
-        reinterpret_cast<longlong2*>(dst)[lb] = make_longlong2(gj, gi);
-        int2* q = reinterpret_cast<int2*>(sh) + 3u * lb;
-        q[0] = make_int2(sx, sy);
-        q[1] = make_int2(sz, -sx);
-        q[2] = make_int2(-sy, -sz);
-        if (A.dist) reinterpret_cast<double2*>(A.dist + cur[g])[lb] = make_double2(d2, d2);
-        if (DISP) {
-          double2* r = reinterpret_cast<double2*>(A.disp + 3 * cur[g]) + 3u * lb;
-          r[0] = make_double2(ex, ey);
-          r[1] = make_double2(ez, -ex);
-          r[2] = make_double2(-ey, -ez);
-        }
-      } else {
-        src[lb] = gi;
-        dst[lb] = gj;
-        int32_t* q = sh + 3u * lb;  // one address, three offsets
-        q[0] = sx;
-        q[1] = sy;
-        q[2] = sz;
-        if (A.dist) A.dist[cur[g] + lb] = d2;  // squared here; square roots in a pass of their own
-        if (DISP) {
-          double* r = A.disp + 3 * cur[g] + 3u * lb;
-          r[0] = ex;
-          r[1] = ey;
-          r[2] = ez;
-        }
+    for (int g = 0; g < NQ; ++g) {
+      if (c[g] >= kRebase) {
+        src[g] += c[g];
+        dst[g] += c[g];
+        sh[g] += 3 * (int64_t)c[g];
+        if (DIST) dist[g] += c[g];
+        if (DISP) disp[g] += 3 * (int64_t)c[g];
+        c[g] = 0;
       }
     }
-    cur[g] += (FULL ? 2 : 1) * (int64_t)__popcll(m);
   }
+};
+
+template <bool FILL, int NQ, bool DIST, bool DISP>
+using Sink = typename std::conditional<FILL, Out<NQ, DIST, DISP>, Counts<NQ>>::type;
+
+// Writes the pairs of the lanes in m, in lane order, at list g's cursor:
+// (gi, gj, S) in a half list; (gi, gj, S) and (gj, gi, -S) at two
+// consecutive slots in a full list (so every count is even and a pair goes
+// out as 16-byte index stores and three 8-byte shift stores).  Indices are
+// below 2^29 (at most 2^28 atoms per launch), so an index is its 32-bit
+// value with a zero high word.  Plain stores: L2 merges the partial lines of
+// successive steps (non-temporal stores measured 10% slower here).
+template <int NQ, bool FULL, bool DIST, bool DISP>
+__device__ __forceinline__ void put(Out<NQ, DIST, DISP>& O, int g, uint64_t m, uint32_t gi,
+                                    uint32_t gj, int sx, int sy, int sz, double d2, double ex,
+                                    double ey, double ez) {
+  if (m == 0) return;
+  if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+    const uint32_t e = O.c[g] + (FULL ? 2u : 1u) * lanes_below(m);  // this lane's slot
+    const uint32_t e8 = e << 3;
+    char* const s8 = reinterpret_cast<char*>(O.src[g]);
+    char* const d8 = reinterpret_cast<char*>(O.dst[g]);
+    char* const q12 = reinterpret_cast<char*>(O.sh[g]) + __umul24(e, 12u);
+    if (!FULL) {
+      *reinterpret_cast<int64_t*>(s8 + e8) = (int64_t)gi;
+      *reinterpret_cast<int64_t*>(d8 + e8) = (int64_t)gj;
+      int32_t* q = reinterpret_cast<int32_t*>(q12);  // one address, three offsets
+      q[0] = sx;
+      q[1] = sy;
+      q[2] = sz;
+      // squared here; square roots in a pass of their own
+      if (DIST) *reinterpret_cast<double*>(reinterpret_cast<char*>(O.dist[g]) + e8) = d2;
+      if (DISP) {
+        double* r = reinterpret_cast<double*>(reinterpret_cast<char*>(O.disp[g]) + __umul24(e, 24u));
+        r[0] = ex;
+        r[1] = ey;
+        r[2] = ez;
+      }
+    } else {
+      *reinterpret_cast<longlong2*>(s8 + e8) = make_longlong2((int64_t)gi, (int64_t)gj);
+      *reinterpret_cast<longlong2*>(d8 + e8) = make_longlong2((int64_t)gj, (int64_t)gi);
+      int2* q = reinterpret_cast<int2*>(q12);
+      q[0] = make_int2(sx, sy);
+      q[1] = make_int2(sz, -sx);
+      q[2] = make_int2(-sy, -sz);
+      if (DIST)
+        *reinterpret_cast<double2*>(reinterpret_cast<char*>(O.dist[g]) + e8) = make_double2(d2, d2);
+      if (DISP) {
+        double2* r = reinterpret_cast<double2*>(reinterpret_cast<char*>(O.disp[g]) + __umul24(e, 24u));
+        r[0] = make_double2(ex, ey);
+        r[1] = make_double2(ez, -ex);
+        r[2] = make_double2(-ey, -ez);
+      }
+    }
+  }
+  O.c[g] += (FULL ? 2u : 1u) * (uint32_t)__popcll(m);
 }
 
-template <bool FILL, int NQ, bool FULL, bool DISP>
-__device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t u,
-                                int mode, Acc<FILL>* cur) {
+// Brute-force row i of a system (one image per pair, i < j in row-major
+// order, search.rs:1187-1405).
+template <bool FILL, int NQ, bool FULL, bool DIST, bool DISP>
+__device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C,
+                                                const SysDesc* d, int64_t u, int mode,
+                                                Sink<FILL, NQ, DIST, DISP>& S) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int n = d->n_atoms;
   const int i = (int)(u - a0);
   const double* pi = A.pos + 3 * u;
   const double xi = pi[0], yi = pi[1], zi = pi[2];
-  const int64_t gi = A.global_index ? u : (int64_t)i;
+  const uint32_t gi = (uint32_t)(A.global_index ? u : (int64_t)i);
   const bool pb0 = d->pbc[0], pb1 = d->pbc[1], pb2 = d->pbc[2];
   double hv[9], hiv[9];
 #pragma unroll
@@ -889,10 +945,25 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
       }
       d2 = norm2(dx, dy, dz);
     }
-    const uint32_t mem = valid ? members64_rt(d2, C) : 0u;
-    if (FILL && __ballot(mem != 0) == 0) continue;
-    const int64_t gj = A.global_index ? a0 + j : (int64_t)j;
-    emit_step<FILL, NQ, FULL, DISP>(A, C, mem, gi, gj, sx, sy, sz, d2, dx, dy, dz, cur);
+    uint64_t m[NQ];
+    if (C.disjoint)
+      lists64<true, NQ>(d2, C, ~0ull, m);
+    else
+      lists64<false, NQ>(d2, C, ~0ull, m);
+    if constexpr (FILL) {
+      uint64_t any = 0;
+#pragma unroll
+      for (int g = 0; g < NQ; ++g) any |= m[g];
+      if (any == 0) continue;
+      S.rebase();
+      const uint32_t gj = (uint32_t)(A.global_index ? a0 + j : (int64_t)j);
+#pragma unroll
+      for (int g = 0; g < NQ; ++g)
+        put<NQ, FULL, DIST, DISP>(S, g, m[g], gi, gj, sx, sy, sz, d2, dx, dy, dz);
+    } else {
+#pragma unroll
+      for (int g = 0; g < NQ; ++g) S.c[g] += (uint32_t)__popcll(m[g]);
+    }
   }
 }
 
@@ -921,9 +992,13 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 // orientation (lower original index first, S = s_j - s_i + bs from it,
 // search.rs:695-697); a self image (same atom, bs != 0) is written for both
 // S and -S, as the reference emits both (search.rs:549).
-template <bool FILL, int NQ, bool F32, bool FULL, bool DISP>
-__device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d, int64_t p,
-                          int part, WaveLds& L, Acc<FILL>* cur) {
+// Kinds of candidate chunk (see the home-atom loop of unit_cell).
+constexpr int kPlain = 0, kSelf = 1, kHome = 2;
+
+template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
+__device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d,
+                                          int64_t p, int part, WaveLds& L,
+                                          Sink<FILL, NQ, DIST, DISP>& S) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int nbx = d->nb[0], nby = d->nb[1], nbz = d->nb[2];
@@ -960,6 +1035,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   const bool diag = d->h[1] == 0.0 && d->h[2] == 0.0 && d->h[3] == 0.0 && d->h[5] == 0.0 &&
                     d->h[6] == 0.0 && d->h[7] == 0.0;
   const float prune2 = (float)(d->rc_max * d->rc_max * (1.0 + 1e-4));
+  const int64_t base_idx = A.global_index ? a0 : 0;
   for (int hb = hs; hb < he; hb += 64) {
     const int hn = uniform_i32(min(64, he - hb));
     if (lane < hn) {
@@ -1093,6 +1169,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             locate(c0 + 64);
             fetch();
           }
+          if constexpr (FILL) S.rebase();
           const bool home_seg = (t0 == tc) && (csg == 0);
           const double cox = L.off[0][csg], coy = L.off[1][csg], coz = L.off[2][csg];
           const int sbx = L.bs[0][csg], sby = L.bs[1][csg], sbz = L.bs[2][csg];
@@ -1113,18 +1190,20 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
           // fill pass, per chunk: the candidate's part of S (s_j + bs) and its
           // emitted index
           const int tjx = cj.x + sbx, tjy = cj.y + sby, tjz = cj.z + sbz;
-          const int64_t base_idx = A.global_index ? a0 : 0;
-          const int64_t gj_c = base_idx + oj;
+          const uint32_t gj_c = (uint32_t)(base_idx + oj);
           // home atom i is tested against this lane's candidate iff i < lim
           // (in the home bin without image only the later slots)
           const int lim = cvalid ? (home_seg ? cjs - hb : 64) : 0;
-          // the home-atom loop, compiled once per disjoint mode (a uniform
-          // branch inside it costs register moves and selects on every step)
-          // and once for chunks where every lane tests every home atom: no
-          // home-bin lanes (slot limit) and no own images possible
-          auto home_loop = [&](auto disj_tag, auto simple_tag) {
+          // The home-atom loop, compiled per disjoint mode and per chunk kind:
+          //   kPlain  every lane tests every home atom (no home-bin lanes,
+          //           no own images possible)
+          //   kSelf   own images possible: a step where a lane holds an
+          //           image of the home atom counts / writes its -S copy
+          //   kHome   home-bin lanes too: lanes at or before the home atom's
+          //           slot get d^2 = +inf (no list takes them)
+          auto home_loop = [&](auto disj_tag, auto kind_tag) {
             constexpr bool DISJ = decltype(disj_tag)::value;
-            constexpr bool SIMPLE = decltype(simple_tag)::value;
+            constexpr int KIND = decltype(kind_tag)::value;
 #pragma unroll 1
             for (int i = 0; i < hn; ++i) {
               // broadcast LDS reads of the home atom (uniform address)
@@ -1135,125 +1214,111 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
               else
                 h64 = L.home64[i];
               const int oi = F32 ? __float_as_int(h32.w) : __double2loint(h64.w);
-              const bool cand = SIMPLE || i < lim;
-              if (!FILL && !DISJ) {
-                // Count pass, cumulative lists: lane counters, no ballots.  A
-                // self image (same atom, other image) counts twice in a half
-                // list (both S and -S are emitted); full lists count every hit
-                // once and double the item's count at the end.  Lists past
-                // q_cnt have -inf thresholds.
-                const uint32_t w1 =
-                    SIMPLE ? 1u : (cand ? 1u + (uint32_t)(!FULL && oj == oi) : 0u);
-                if (F32) {
-                  const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
-                  const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                  bool unsure = false;
-#pragma unroll
-                  for (int g = 0; g < NQ; ++g) {
-                    const bool acc = e2 < C.lo[g];
-                    cur[g] += acc ? w1 : 0u;
-                    unsure |= !acc && e2 <= C.hi[g];
-                  }
-                  unsure = unsure && cand;
-                  if (__ballot(unsure)) {
-                    const Pos64 h64 = A.spos64[hb + i];  // uniform address
-                    const double xi64 = h64.x, yi64 = h64.y, zi64 = h64.z;
-                    if (unsure) {
-                      // FP64 recheck: the reference expression, exactly
-                      const Pos64 c64 = A.spos64[cjs];
-                      const double d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy,
-                                              (c64.z - zi64) + coz);
-#pragma unroll
-                      for (int g = 0; g < NQ; ++g)
-                        cur[g] += (!(e2 < C.lo[g]) && d2 < C.r[g]) ? w1 : 0u;
-                    }
-                  }
-                } else {
-                  // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-                  const double d2 =
-                      norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
-#pragma unroll
-                  for (int g = 0; g < NQ; ++g) cur[g] += d2 < C.r[g] ? w1 : 0u;
-                }
-                continue;
-              }
-              uint32_t mem = 0;
-              double d2 = INFINITY;
+              const bool cand = KIND != kHome || i < lim;
+              uint64_t m[NQ];
+              double d2 = INFINITY, ex = 0.0, ey = 0.0, ez = 0.0;
               if (F32) {
                 const float dx = px - h32.x, dy = py - h32.y, dz = pz - h32.z;
-                const float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                bool unsure = false;
-                mem = members32<DISJ, NQ>(e2, C, &unsure);
-                mem = cand ? mem : 0u;
-                const bool need64 =
-                    cand && (unsure || ((DISP || A.dist != nullptr) && mem != 0));
-                if (__ballot(need64)) {
-                  const Pos64 h64 = A.spos64[hb + i];  // uniform address
-                  const double xi64 = h64.x, yi64 = h64.y, zi64 = h64.z;
-                  if (need64) {
+                float e2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (KIND == kHome && !cand) e2 = INFINITY;
+                uint64_t lt[NQ], below;
+                uint64_t need = filter32<DISJ, NQ>(e2, C, lt, &below);
+                if (FILL && (DIST || DISP)) {
+                  // exact distances of every pair taken
+#pragma unroll
+                  for (int g = 0; g < NQ; ++g) need |= lt[g];
+                }
+                if (need) {
+                  const Pos64 hh = A.spos64[hb + i];  // uniform address
+                  if (__builtin_amdgcn_inverse_ballot_w64(need)) {
                     // FP64 recheck: the reference expression, exactly
-                    const Pos64 c64 = A.spos64[cjs];
-                    d2 = norm2((c64.x - xi64) + cox, (c64.y - yi64) + coy, (c64.z - zi64) + coz);
-                    mem = members64<DISJ, NQ>(d2, C);
+                    const Pos64 cc = A.spos64[cjs];
+                    ex = (cc.x - hh.x) + cox;
+                    ey = (cc.y - hh.y) + coy;
+                    ez = (cc.z - hh.z) + coz;
+                    d2 = norm2(ex, ey, ez);
+                  }
+                  recheck64<DISJ, NQ>(d2, C, need, lt, &below);
+                }
+                shells<DISJ, NQ>(lt, below, ~0ull, m);
+              } else {
+                // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
+                ex = (c.x - h64.x) + cox;
+                ey = (c.y - h64.y) + coy;
+                ez = (c.z - h64.z) + coz;
+                d2 = norm2(ex, ey, ez);
+                if (KIND == kHome && !cand) d2 = INFINITY;
+                lists64<DISJ, NQ>(d2, C, ~0ull, m);
+              }
+              if constexpr (!FILL) {
+                // A self image (same atom, other image) counts twice in a
+                // half list (both S and -S are emitted); full lists count every
+                // hit once and double the item's count at the end.
+#pragma unroll
+                for (int g = 0; g < NQ; ++g) S.c[g] += (uint32_t)__popcll(m[g]);
+                if (KIND != kPlain && !FULL) {
+                  const uint64_t self = __ballot(oj == oi);
+                  if (self) {
+#pragma unroll
+                    for (int g = 0; g < NQ; ++g) S.c[g] += (uint32_t)__popcll(m[g] & self);
                   }
                 }
               } else {
-                // search.rs:540-543: d = (p_j - p_i) + offset, d^2 = (dx^2 + dy^2) + dz^2
-                d2 = norm2((c.x - h64.x) + cox, (c.y - h64.y) + coy, (c.z - h64.z) + coz);
-                if (cand) mem = members64<DISJ, NQ>(d2, C);
-              }
-              if (FILL && __ballot(mem != 0) == 0) continue;
-              const bool self_img = !SIMPLE && (mem != 0) && (oj == oi);
-              int sx = 0, sy = 0, sz = 0;
-              int64_t gi = 0, gj = 0;
-              if (FILL) {
-                const int4 hsi = L.homesi[i];
+                uint64_t any = 0;
+#pragma unroll
+                for (int g = 0; g < NQ; ++g) any |= m[g];
+                if (any == 0) continue;
                 // S = s_j - s_i + bs in the reference orientation (lower
                 // original index first): negated when the candidate is lower
+                const int4 hsi = L.homesi[i];
                 const bool flip = oj < oi;
                 const int sgn = flip ? -1 : 0;  // x -> (x ^ sgn) - sgn negates when flipped
-                sx = ((tjx - hsi.x) ^ sgn) - sgn;
-                sy = ((tjy - hsi.y) ^ sgn) - sgn;
-                sz = ((tjz - hsi.z) ^ sgn) - sgn;
-                const int64_t gi_h = base_idx + oi;  // uniform
-                gi = flip ? gj_c : gi_h;
-                gj = flip ? gi_h : gj_c;
+                const int sx = ((tjx - hsi.x) ^ sgn) - sgn;
+                const int sy = ((tjy - hsi.y) ^ sgn) - sgn;
+                const int sz = ((tjz - hsi.z) ^ sgn) - sgn;
+                const uint32_t gi_h = (uint32_t)(base_idx + oi);
+                const uint32_t gi = flip ? gj_c : gi_h;
+                const uint32_t gj = flip ? gi_h : gj_c;
+                // displacement in the edge's orientation
+                if (DISP && flip) {
+                  ex = -ex;
+                  ey = -ey;
+                  ez = -ez;
+                }
+#pragma unroll
+                for (int g = 0; g < NQ; ++g)
+                  put<NQ, FULL, DIST, DISP>(S, g, m[g], gi, gj, sx, sy, sz, d2, ex, ey, ez);
+                // the self image's -S copy (full lists get it from the mirror)
+                if (KIND != kPlain && !FULL) {
+                  const uint64_t self = __ballot(oj == oi) & any;
+                  if (self) {
+#pragma unroll
+                    for (int g = 0; g < NQ; ++g)
+                      put<NQ, false, DIST, DISP>(S, g, m[g] & self, gi, gj, -sx, -sy, -sz, d2,
+                                                 -ex, -ey, -ez);
+                  }
+                }
               }
-              // displacement in the edge's orientation (lower original index
-            // first), recomputed from the operands of d^2 only when asked for
-            double ex = 0.0, ey = 0.0, ez = 0.0;
-            if (DISP && mem != 0) {
-              const Pos64 cc = F32 ? A.spos64[cjs] : c;
-              const Pos64 hh = F32 ? A.spos64[hb + i] : h64;
-              ex = (cc.x - hh.x) + cox;
-              ey = (cc.y - hh.y) + coy;
-              ez = (cc.z - hh.z) + coz;
-              if (oj < oi) {
-                ex = -ex;
-                ey = -ey;
-                ez = -ez;
-              }
-            }
-            emit_step<FILL, NQ, FULL, DISP>(A, C, mem, gi, gj, sx, sy, sz, d2, ex, ey, ez, cur);
-              // the self image's -S copy (full lists get it from the mirror;
-              // their count pass counts each half-stencil hit once, doubled
-              // when the item ends)
-              if (!FULL && !SIMPLE && (!FILL || __ballot(self_img)))
-                emit_step<FILL, NQ, false, DISP>(A, C, self_img ? mem : 0u, gi, gj, -sx, -sy, -sz,
-                                                 d2, -ex, -ey, -ez, cur);
             }
           };
-          const bool simple = !self_possible && __ballot(home_seg) == 0;
+          using KPlain = std::integral_constant<int, kPlain>;
+          using KSelf = std::integral_constant<int, kSelf>;
+          using KHome = std::integral_constant<int, kHome>;
+          const bool homes = __ballot(home_seg) != 0;
           if (C.disjoint) {
-            if (simple)
-              home_loop(std::true_type{}, std::true_type{});
+            if (homes)
+              home_loop(std::true_type{}, KHome{});
+            else if (self_possible)
+              home_loop(std::true_type{}, KSelf{});
             else
-              home_loop(std::true_type{}, std::false_type{});
+              home_loop(std::true_type{}, KPlain{});
           } else {
-            if (simple)
-              home_loop(std::false_type{}, std::true_type{});
+            if (homes)
+              home_loop(std::false_type{}, KHome{});
+            else if (self_possible)
+              home_loop(std::false_type{}, KSelf{});
             else
-              home_loop(std::false_type{}, std::false_type{});
+              home_loop(std::false_type{}, KPlain{});
           }
         }
       }
@@ -1270,7 +1335,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
 // takes the cell-list path; slots that do not start a group are empty).
 // Edges of one system are thus contiguous and systems come out in input
 // order, as batch.rs:135-144 concatenates them.
-template <bool FILL, int NQ, bool F32, bool FULL, bool DISP>
+template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
 __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds& L) {
   const int s = uniform_i32(find_sys(A.item_base, A.n_sys, u));
   const SysDesc* d = A.sys + s;
@@ -1286,7 +1351,7 @@ __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds&
   const float2 bp = A.q_lo > 0 ? band[A.q_lo - 1] : make_float2(-INFINITY, -INFINITY);
   C.prev_lo = bp.x;
   C.prev_hi = bp.y;
-  Acc<FILL> cur[NQ];
+  int64_t e0[NQ];
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
     const bool on = g < A.q_cnt;
@@ -1297,29 +1362,33 @@ __device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds&
     C.hi[g] = on ? b.y : -INFINITY;
     C.base[g] = FILL && on ? A.bounds[q] : 0;
     C.len[g] = FILL && on ? A.bounds[q + 1] - A.bounds[q] : 0;
-    cur[g] = (FILL && on) ? A.offsets[(int64_t)q * A.n + u] : 0;
+    e0[g] = (FILL && on) ? A.offsets[(int64_t)q * A.n + u] : 0;
+  }
+  Sink<FILL, NQ, DIST, DISP> S;
+  if constexpr (FILL) {
+    S.init(A, C, e0);
+  } else {
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) S.c[g] = 0;
   }
   const int n_atoms = d->n_atoms;
   if (local < n_atoms) {
     if (mode != kCellList)
-      unit_bruteforce<FILL, NQ, FULL, DISP>(A, C, d, d->atom_start + local, mode, cur);
+      unit_bruteforce<FILL, NQ, FULL, DIST, DISP>(A, C, d, d->atom_start + local, mode, S);
   } else if (mode == kCellList) {
     const int64_t r = local - n_atoms;
     const int64_t p = d->atom_start + r / A.parts;
-    unit_cell<FILL, NQ, F32, FULL, DISP>(A, C, d, p, (int)(r % A.parts), L, cur);
+    unit_cell<FILL, NQ, F32, FULL, DIST, DISP>(A, C, d, p, (int)(r % A.parts), L, S);
   }
-  if (!FILL) {
+  if constexpr (!FILL) {
+    // the counts are wave-uniform: lane 0 writes them
 #pragma unroll
     for (int g = 0; g < NQ; ++g) {
-      uint32_t t = cur[g];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-      if (FULL) t *= 2u;  // both directions of every half-stencil hit
+      const uint32_t t = FULL ? 2u * S.c[g] : S.c[g];  // both directions of every hit
       if (g < A.q_cnt && lane_id() == 0) A.counts[(int64_t)(A.q_lo + g) * A.n + u] = t;
     }
   }
 }
-
 // Work list: the items that have work, appended with one atomic per wave
 // and kind.  Home groups of cell-list systems (long: a whole half stencil)
 // fill the list from the front, brute-force rows (short) from the back, so
@@ -1378,35 +1447,57 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 // One wave per listed work item (a loop over several items per wave keeps
 // the per-item state live across iterations and costs about 40 VGPRs).
 // F32: FP32 filter with FP64 recheck in the band (cell-list items).
-// FULL: full lists (count and fill passes).  DISP: the fill pass also
-// stores displacements.
-template <bool FILL, int NQ, bool F32, bool FULL, bool DISP>
+// FULL: full lists (count and fill passes).  DIST / DISP: the fill pass
+// also stores squared distances / displacements.
+//
+// Blocks are dispatched round-robin over the 8 XCDs (block b to XCD b % 8),
+// each with its own L2.  With xcd_chunk = c > 0 the work list is dealt to
+// the XCDs in runs of c blocks (XCD x takes runs x, x + 8, ...), so
+// neighbouring home groups, which read the same stencil bins, run under one
+// L2 while every XCD still sees the whole mix of long and short items;
+// c = 0 keeps the dispatch order.  Only the blocks that have an item take
+// part; the others leave at once.
+template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
 __device__ __forceinline__ void pairs_body(const PairArgs& A) {
   __shared__ WaveLds s_lds[kWaves];
   const int w = uniform_i32(threadIdx.x >> 6);
-  const int64_t k = (int64_t)blockIdx.x * kWaves + w;
   const int64_t front = A.n_real[0];
-  if (k >= front + (int64_t)A.n_real[1]) return;
+  const int64_t total = front + (int64_t)A.n_real[1];
+  const int64_t nblk = (total + kWaves - 1) / kWaves;
+  const int64_t b = blockIdx.x;
+  if (b >= nblk) return;
+  constexpr int64_t kXcd = 8;
+  int64_t bb = b;
+  const int64_t ch = A.xcd_chunk;
+  if (ch > 0) {
+    const int64_t span = kXcd * ch;
+    if (b < nblk / span * span) {  // the tail past whole spans keeps its order
+      const int64_t q = b / kXcd;
+      bb = (q / ch) * span + (b % kXcd) * ch + q % ch;
+    }
+  }
+  const int64_t k = bb * kWaves + w;
+  if (k >= total) return;
   // wave-uniform item: its loads are scalar and the fill cursors stay scalar
   const int64_t u = uniform_i64(k < front ? A.list[k] : A.list[A.n - 1 - (k - front)]);
   // fill pass launched before the host saw the counts: write nothing when
   // the lists do not fit; the host re-runs the fill with exact buffers
   if (FILL && A.bounds[A.k] > A.capacity) return;
-  pair_item<FILL, NQ, F32, FULL, DISP>(A, u, s_lds[w]);
+  pair_item<FILL, NQ, F32, FULL, DIST, DISP>(A, u, s_lds[w]);
 }
 
-template <bool FILL, int NQ, bool F32, bool FULL, bool DISP = false>
+template <bool FILL, int NQ, bool F32, bool FULL, bool DIST = false, bool DISP = false>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
-  pairs_body<FILL, NQ, F32, FULL, DISP>(A);
+  pairs_body<FILL, NQ, F32, FULL, DIST, DISP>(A);
 }
 
 // The same kernel held to 80 VGPRs (6 waves per SIMD).  Used for the FP64
-// path without the full-list fill, where that fits without scratch and
-// measured a few per cent faster; the other variants would spill.
+// path without the full-list fill and without distances or displacements,
+// where that fits without scratch and measured a few per cent faster.
 template <bool FILL, int NQ, bool F32, bool FULL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_w6(
     PairArgs A) {
-  pairs_body<FILL, NQ, F32, FULL, false>(A);
+  pairs_body<FILL, NQ, F32, FULL, false, false>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -1494,22 +1585,19 @@ template <int NQ, bool FULL>
 __device__ __forceinline__ void small_count_row(const PairArgs& A, const Cuts<NQ>& C,
                                                 const SysDesc* d, int i, int mode, int k,
                                                 int n, uint32_t* out) {
-  Acc<false> cur[NQ];
+  Counts<NQ> cnt;
 #pragma unroll
-  for (int g = 0; g < NQ; ++g) cur[g] = 0;
-  unit_bruteforce<false, NQ, FULL, false>(A, C, d, (int64_t)i, mode, cur);
+  for (int g = 0; g < NQ; ++g) cnt.c[g] = 0;
+  unit_bruteforce<false, NQ, FULL, false, false>(A, C, d, (int64_t)i, mode, cnt);
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
-    uint32_t t = cur[g];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if (FULL) t *= 2u;
+    const uint32_t t = FULL ? 2u * cnt.c[g] : cnt.c[g];  // wave-uniform
     if (g < k && lane_id() == 0) out[g * n + i] = t;
   }
 }
 
 // One block: plan, per-row counts, scan, fill.
-template <int NQ, bool FULL, bool DISP>
+template <int NQ, bool FULL, bool DIST, bool DISP>
 __global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
   __shared__ SysDesc s_d;
   __shared__ double s_rc2[kGroup];
@@ -1557,12 +1645,14 @@ __global__ __launch_bounds__(1024) void k_small(SmallArgs a) {
     C.len[g] = g < k ? s_bounds[g + 1] - s_bounds[g] : 0;
   }
   for (int i = w; i < n; i += 16) {
-    Acc<true> cur[NQ];
+    int64_t e0[NQ];
 #pragma unroll
-    for (int g = 0; g < NQ; ++g) cur[g] = g < k ? (int64_t)s_off[g * n + i] : 0;
-    unit_bruteforce<true, NQ, FULL, DISP>(A, C, &s_d, (int64_t)i, mode, cur);
+    for (int g = 0; g < NQ; ++g) e0[g] = g < k ? (int64_t)s_off[g * n + i] : 0;
+    Out<NQ, DIST, DISP> O;
+    O.init(A, C, e0);
+    unit_bruteforce<true, NQ, FULL, DIST, DISP>(A, C, &s_d, (int64_t)i, mode, O);
   }
-  if (a.dist) {
+  if (DIST) {
     // the fill stored squared distances (as the pipeline's fill pass); the
     // block's stores are visible to it after the barrier
     __syncthreads();
@@ -1626,7 +1716,7 @@ __global__ __launch_bounds__(1024) void k_brute_count(SmallArgs a) {
 
 // Fill launch: the rows of the count launch at the scanned offsets; nothing
 // when the plan declined or the lists exceed the outputs.
-template <int NQ, bool FULL, bool DISP>
+template <int NQ, bool FULL, bool DIST, bool DISP>
 __global__ __launch_bounds__(1024) void k_brute_fill(SmallArgs a) {
   __shared__ SysDesc s_d;
   __shared__ double s_pos[3 * kSmallRows];
@@ -1643,14 +1733,16 @@ __global__ __launch_bounds__(1024) void k_brute_fill(SmallArgs a) {
   if (i >= n) return;
   const PairArgs A = small_pair_args(a, s_pos);
   Cuts<NQ> C = small_cuts<NQ>(a.rc2, k, a.disjoint);
-  Acc<true> cur[NQ];
+  int64_t e0[NQ];
 #pragma unroll
   for (int g = 0; g < NQ; ++g) {
     C.base[g] = g < k ? a.bounds[g] : 0;
     C.len[g] = g < k ? a.bounds[g + 1] - a.bounds[g] : 0;
-    cur[g] = g < k ? (int64_t)a.offs[g * n + i] : 0;
+    e0[g] = g < k ? (int64_t)a.offs[g * n + i] : 0;
   }
-  unit_bruteforce<true, NQ, FULL, DISP>(A, C, &s_d, (int64_t)i, s_d.mode, cur);
+  Out<NQ, DIST, DISP> O;
+  O.init(A, C, e0);
+  unit_bruteforce<true, NQ, FULL, DIST, DISP>(A, C, &s_d, (int64_t)i, s_d.mode, O);
 }
 
 // ---------------------------------------------------------------------------
@@ -1953,46 +2045,58 @@ hipError_t launch_finalize(const uint32_t* keys, const uint32_t* vals, const dou
   return hipGetLastError();
 }
 
+// Fill kernels by (FP32 filter, full lists, distances, displacements); the
+// plain FP64 half-list fill is the 6-waves-per-SIMD build.
+template <int NQ, bool F32, bool FULL, bool DIST, bool DISP>
+static void fill_launch(const dim3& g, const dim3& b, const PairArgs& a, hipStream_t st) {
+  if constexpr (!F32 && !FULL && !DIST && !DISP)
+    hipLaunchKernelGGL((k_pairs_w6<true, NQ, false, false>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_pairs<true, NQ, F32, FULL, DIST, DISP>), g, b, 0, st, a);
+}
+template <int NQ, bool F32, bool FULL, bool DIST>
+static void fill_disp(const dim3& g, const dim3& b, const PairArgs& a, hipStream_t st) {
+  if (a.disp)
+    fill_launch<NQ, F32, FULL, DIST, true>(g, b, a, st);
+  else
+    fill_launch<NQ, F32, FULL, DIST, false>(g, b, a, st);
+}
+template <int NQ, bool F32, bool FULL>
+static void fill_dist(const dim3& g, const dim3& b, const PairArgs& a, hipStream_t st) {
+  if (a.dist)
+    fill_disp<NQ, F32, FULL, true>(g, b, a, st);
+  else
+    fill_disp<NQ, F32, FULL, false>(g, b, a, st);
+}
+template <int NQ, bool F32>
+static void fill_full(const dim3& g, const dim3& b, const PairArgs& a, hipStream_t st) {
+  if (a.full)
+    fill_dist<NQ, F32, true>(g, b, a, st);
+  else
+    fill_dist<NQ, F32, false>(g, b, a, st);
+}
+
 template <int NQ>
 static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
   const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
-#define NL_PAIRS(KERNEL, FILL, F32, FULL) \
-  hipLaunchKernelGGL((KERNEL<FILL, NQ, F32, FULL>), g, b, 0, st, a)
-#define NL_PAIRS_DISP(F32, FULL) \
-  hipLaunchKernelGGL((k_pairs<true, NQ, F32, FULL, true>), g, b, 0, st, a)
-  if (fill && a.disp) {  // fill pass that also stores displacements
-    if (a.use_fp32) {
-      if (a.full)
-        NL_PAIRS_DISP(true, true);
-      else
-        NL_PAIRS_DISP(true, false);
-    } else {
-      if (a.full)
-        NL_PAIRS_DISP(false, true);
-      else
-        NL_PAIRS_DISP(false, false);
-    }
-  } else if (a.use_fp32) {
-    if (fill && a.full)
-      NL_PAIRS(k_pairs, true, true, true);
-    else if (fill)
-      NL_PAIRS(k_pairs, true, true, false);
-    else if (a.full)
-      NL_PAIRS(k_pairs, false, true, true);
+  if (fill) {
+    if (a.use_fp32)
+      fill_full<NQ, true>(g, b, a, st);
     else
-      NL_PAIRS(k_pairs, false, true, false);
-  } else {
-    if (fill && a.full)
-      NL_PAIRS(k_pairs, true, false, true);
-    else if (fill)
-      NL_PAIRS(k_pairs_w6, true, false, false);
-    else if (a.full)
-      NL_PAIRS(k_pairs_w6, false, false, true);
-    else
-      NL_PAIRS(k_pairs_w6, false, false, false);
+      fill_full<NQ, false>(g, b, a, st);
+    return;
   }
-#undef NL_PAIRS_DISP
-#undef NL_PAIRS
+  if (a.use_fp32) {
+    if (a.full)
+      hipLaunchKernelGGL((k_pairs<false, NQ, true, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_pairs<false, NQ, true, false>), g, b, 0, st, a);
+  } else {
+    if (a.full)
+      hipLaunchKernelGGL((k_pairs_w6<false, NQ, false, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_pairs_w6<false, NQ, false, false>), g, b, 0, st, a);
+  }
 }
 
 hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const SysDesc* sys,
@@ -2004,41 +2108,53 @@ hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const S
   return hipGetLastError();
 }
 
+template <int NQ, bool FULL, bool DIST>
+static void small_disp(const SmallArgs& a, hipStream_t st) {
+  if (a.disp)
+    hipLaunchKernelGGL((k_small<NQ, FULL, DIST, true>), dim3(1), dim3(1024), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_small<NQ, FULL, DIST, false>), dim3(1), dim3(1024), 0, st, a);
+}
 template <int NQ>
 static void launch_small_nq(const SmallArgs& a, hipStream_t st) {
-  const bool disp = a.disp != nullptr;
   if (a.full) {
-    if (disp)
-      hipLaunchKernelGGL((k_small<NQ, true, true>), dim3(1), dim3(1024), 0, st, a);
+    if (a.dist)
+      small_disp<NQ, true, true>(a, st);
     else
-      hipLaunchKernelGGL((k_small<NQ, true, false>), dim3(1), dim3(1024), 0, st, a);
+      small_disp<NQ, true, false>(a, st);
   } else {
-    if (disp)
-      hipLaunchKernelGGL((k_small<NQ, false, true>), dim3(1), dim3(1024), 0, st, a);
+    if (a.dist)
+      small_disp<NQ, false, true>(a, st);
     else
-      hipLaunchKernelGGL((k_small<NQ, false, false>), dim3(1), dim3(1024), 0, st, a);
+      small_disp<NQ, false, false>(a, st);
   }
 }
 
+template <int NQ, bool FULL, bool DIST>
+static void brute_fill_disp(const dim3& g, const dim3& b, const SmallArgs& a, hipStream_t st) {
+  if (a.disp)
+    hipLaunchKernelGGL((k_brute_fill<NQ, FULL, DIST, true>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_brute_fill<NQ, FULL, DIST, false>), g, b, 0, st, a);
+}
 template <int NQ>
 static void launch_brute_nq(bool fill, const SmallArgs& a, hipStream_t st) {
   const dim3 g((unsigned)((a.desc.n_atoms + 15) / 16)), b(1024);
-  const bool disp = a.disp != nullptr;
   if (!fill) {
     if (a.full)
       hipLaunchKernelGGL((k_brute_count<NQ, true>), g, b, 0, st, a);
     else
       hipLaunchKernelGGL((k_brute_count<NQ, false>), g, b, 0, st, a);
   } else if (a.full) {
-    if (disp)
-      hipLaunchKernelGGL((k_brute_fill<NQ, true, true>), g, b, 0, st, a);
+    if (a.dist)
+      brute_fill_disp<NQ, true, true>(g, b, a, st);
     else
-      hipLaunchKernelGGL((k_brute_fill<NQ, true, false>), g, b, 0, st, a);
+      brute_fill_disp<NQ, true, false>(g, b, a, st);
   } else {
-    if (disp)
-      hipLaunchKernelGGL((k_brute_fill<NQ, false, true>), g, b, 0, st, a);
+    if (a.dist)
+      brute_fill_disp<NQ, false, true>(g, b, a, st);
     else
-      hipLaunchKernelGGL((k_brute_fill<NQ, false, false>), g, b, 0, st, a);
+      brute_fill_disp<NQ, false, false>(g, b, a, st);
   }
 }
 
