@@ -58,6 +58,30 @@ __device__ inline int find_sys(const int64_t* sys_start, int n_sys, int64_t a) {
   return lo;
 }
 
+// Inclusive scan of one 32-bit value per lane over the wave, by DPP lane
+// moves (row shifts 1, 2, 4, 8 inside each row of 16 lanes, then the row
+// broadcasts of lanes 15 and 31): no shuffle addresses in registers.
+__device__ __forceinline__ int wave_incl_max_dpp(int v) {
+  // as wave_incl_scan_dpp, with max; lanes without a source keep -1
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return v;
+}
+
+__device__ __forceinline__ int wave_incl_scan_dpp(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 template <typename T>
 __device__ inline T wave_incl_scan(T v) {
   const int lane = lane_id();
@@ -692,6 +716,7 @@ struct WaveLds {
     float4 home32[64];  // FP32 wrapped position, w = local index bits (FP32 kernels)
   };
   int4 homesi[64];      // wrap shifts, w = local index (fill pass)
+  int32_t mark[64];     // candidate lane -> segment starting there (-1: none)
 };
 
 // q = v / m, r = v % m for 0 <= v < 2^22, 1 <= m < 2^22: a float estimate
@@ -1099,7 +1124,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
         oy = (h3 * (double)bsx + h4 * (double)bsy) + h5 * (double)bsz;
         oz = (h6 * (double)bsx + h7 * (double)bsy) + h8 * (double)bsz;
       }
-      const int incl = wave_incl_scan(len);
+      const int incl = wave_incl_scan_dpp(len);
       // candidate c of this segment list sits at sorted slot base[seg] + c
       const int base = st - (incl - len);
       L.bs[0][lane] = bsx;
@@ -1131,21 +1156,27 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
         float4 n32 = make_float4(0.f, 0.f, 0.f, 0.f);
         Pos64 n64 = {0.0, 0.0, 0.0, 0.0};
         int4 nij = make_int4(0, 0, 0, 0);
-        // This lane's candidate of the chunk at c0: its segment is the number
-        // of segments whose inclusive count is <= c0 + lane.  Lane s holds
-        // segment s's count, so the segments ending before the chunk are
-        // one ballot, and the few ending inside it are walked in order.
+        // This lane's candidate of the chunk at c0 lies in the last
+        // non-empty segment starting at or before it.  Lane s holds segment
+        // s.  The segment holding candidate c0 is one ballot; each segment
+        // that starts inside the chunk marks its first candidate lane in LDS,
+        // and a running maximum over the lanes (DPP) carries every mark
+        // forward: a fixed cost per chunk, however many segments end in it.
+        const int excl = incl - len;
         auto locate = [&](int c0) {
           const int idx = c0 + lane;
           valid = idx < total;
-          sg = (int)__popcll(__ballot(incl <= c0));
-          uint64_t m = __ballot(incl > c0 && incl <= c0 + 63);
-          while (m) {
-            const int e = (int)__builtin_ctzll(m);
-            sg += lane >= __builtin_amdgcn_readlane(incl, e) - c0 ? 1 : 0;
-            m &= m - 1;
-          }
-          sg = min(sg, 63);  // lanes past the last candidate
+          const uint64_t before = __ballot(len > 0 && excl <= c0);
+          const int s0 = before ? 63 - (int)__builtin_clzll(before) : 0;
+          L.mark[lane] = -1;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (len > 0 && excl > c0 && excl <= c0 + 63) L.mark[excl - c0] = lane;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          sg = max(s0, wave_incl_max_dpp(L.mark[lane]));
           j = __shfl(base, sg, 64) + idx;
         };
         auto fetch = [&]() {
