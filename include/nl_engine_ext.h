@@ -256,6 +256,32 @@ int nl_skin_device(const nl_skin* s, size_t list, const int64_t** edge_index,
 int nl_skin_stats(const nl_skin* s, int64_t* updates, int64_t* rebuilds, double* last_disp);
 void nl_skin_free(nl_skin* s);
 
+/* ---- allocation-free search --------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * or page-locked memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): nl_search_ex on device-resident
+ * positions (options: NL_LOCAL_INDEX only), with its device workspace in a
+ * 64 MiB array of the library (one per GPU, held until the process ends),
+ * host staging in ordinary memory, and outputs in the caller's buffers.  It
+ * calls no allocation routine.  Requests whose workspace exceeds 64 MiB
+ * (roughly 200,000 atoms with three lists) are rejected with an error.
+ *
+ * d_edges holds 2 * capacity int64 and d_shifts 3 * capacity int32 (device
+ * memory).  On return list q (input cutoff order) holds list_len[q] edges:
+ * its edge_index (2, list_len[q]) at d_edges + 2 * list_base[q] and its
+ * shifts (list_len[q], 3) at d_shifts + 3 * list_base[q], the same edges as
+ * nl_search_ex.  The call runs on `stream` (hipStream_t, NULL = the null
+ * stream) after the work queued there and synchronises it before returning.
+ * Returns 0 on success, 2 when the lists exceed the capacity (nothing is
+ * written; list_base / list_len give the sizes), 1 on other errors.  Every
+ * call on a GPU uses that GPU's one workspace: calls are serialised within
+ * the process, and calls that overlap on one GPU from elsewhere would race. */
+int nl_search_noalloc(const double* d_positions, size_t n, const int64_t* sys_sizes,
+                      size_t n_sys, const double* cells, const uint8_t* has_cell,
+                      const uint8_t* pbc, const double* cutoffs, size_t n_cutoffs, int disjoint,
+                      int multi, unsigned options, int64_t* d_edges, int32_t* d_shifts,
+                      int64_t capacity, int64_t* list_base, int64_t* list_len, void* stream);
+
 /* Host-side cell helpers with the reference's arithmetic (no GPU needed):
  * H^-1 by the cofactor formula of Cell::new (cell.rs:18-21; returns nonzero
  * with "Cell matrix is not invertible" when det == 0) and Cell::wrap

@@ -23,7 +23,8 @@ OUT_DIR = os.path.join(HERE, "neighborlist_rs", "lib")
 OUT = os.path.join(OUT_DIR, "libnlmi355.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["nl_kernels.hip", "nl_engine.cpp", "nl_capi.cpp", "nl_server.cpp", "nl_skin.cpp"]
+SOURCES = ["nl_kernels.hip", "nl_engine.cpp", "nl_capi.cpp", "nl_server.cpp", "nl_skin.cpp",
+           "nl_noalloc.cpp"]
 FLAGS = [
     "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math",

@@ -172,6 +172,19 @@ int check_cutoffs(const double* c, size_t k, const char* msg) {
 
 }  // namespace
 
+namespace nl {
+// the request builder and cutoff check of the entry points above, for the
+// allocation-free entry point (nl_noalloc.cpp)
+int capi_sizes_request(const double* pos, size_t n, const int64_t* sizes, size_t n_sys,
+                       const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                       Request& req) {
+  return sizes_request(pos, n, sizes, n_sys, cells, has_cell, pbc, req);
+}
+int capi_check_cutoffs(const double* c, size_t k, const char* msg) {
+  return check_cutoffs(c, k, msg);
+}
+}  // namespace nl
+
 extern "C" {
 
 int nl_search_single(const double* positions, size_t n, const double* h, const uint8_t* pbc,

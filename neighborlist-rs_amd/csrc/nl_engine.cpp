@@ -825,29 +825,46 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
 
 static int lg2(double x) { return x <= 1.0 ? 0 : (int)std::floor(std::log2(x)); }
 
-Tuning Engine::tuning_for(const Request& req) {
+static std::tuple<int, int, int, int, int> tune_key(const Request& req) {
+  double rc_max = 0;
+  for (double x : req.cutoffs) rc_max = std::max(rc_max, x);
+  return std::make_tuple((int)req.cutoffs.size(), (int)req.multi, lg2((double)req.n),
+                         lg2((double)req.sys.size()), lg2(rc_max));
+}
+
+static Tuning explicit_tuning(const Request& req) {
   Tuning t;
   Config& c = config();
   t.cell_div = req.cell_div > 0 ? req.cell_div : c.cell_div.load();
   t.use_fp32 = req.use_fp32 >= 0 ? req.use_fp32 : c.use_fp32.load();
   t.group = req.group > 0 ? req.group : c.group.load();
   t.parts = req.parts > 0 ? req.parts : c.parts.load();
-  const bool explicit_req =
-      req.cell_div > 0 || req.use_fp32 >= 0 || req.group > 0 || req.parts > 0;
-  if (!c.autotune.load() || explicit_req) return t;
-  if (req.n < kTuneMinAtoms) {
+  return t;
+}
+
+static bool is_explicit(const Request& req) {
+  return req.cell_div > 0 || req.use_fp32 >= 0 || req.group > 0 || req.parts > 0;
+}
+
+Tuning untimed_tuning(const Request& req) {
+  Tuning t = explicit_tuning(req);
+  if (config().autotune.load() && !is_explicit(req) && req.n < kTuneMinAtoms) {
     // too small to fill the GPU (and to be worth timing): the launch's
     // latency is its longest work item, so more, smaller items -- two home
     // atoms per group and four stencil slices (measured 1.5x faster than
     // 4 x 1 per single-system call on systems of 8 to 2048 atoms)
     t.group = 2;
     t.parts = 4;
-    return t;
   }
-  double rc_max = 0;
-  for (double x : req.cutoffs) rc_max = std::max(rc_max, x);
-  const auto key = std::make_tuple((int)req.cutoffs.size(), (int)req.multi, lg2((double)req.n),
-                                   lg2((double)req.sys.size()), lg2(rc_max));
+  return t;
+}
+
+Tuning Engine::tuning_for(const Request& req) {
+  Tuning t = explicit_tuning(req);
+  Config& c = config();
+  if (!c.autotune.load() || is_explicit(req)) return t;
+  if (req.n < kTuneMinAtoms) return untimed_tuning(req);
+  const auto key = tune_key(req);
   {
     std::lock_guard<std::mutex> g(tune_mu_);
     auto it = tuned_.find(key);
@@ -924,6 +941,15 @@ Tuning Engine::tuning_for(const Request& req) {
   return best;
 }
 
+Tuning Engine::tuned_or_untimed(const Request& req) {
+  if (config().autotune.load() && !is_explicit(req) && req.n >= kTuneMinAtoms) {
+    std::lock_guard<std::mutex> g(tune_mu_);
+    auto it = tuned_.find(tune_key(req));
+    if (it != tuned_.end()) return it->second;
+  }
+  return untimed_tuning(req);
+}
+
 std::string Engine::tune_report() {
   std::lock_guard<std::mutex> g(tune_mu_);
   std::ostringstream o;
@@ -943,17 +969,26 @@ void Engine::clear_tuning() {
   tuned_.clear();
 }
 
+static std::mutex g_engines_mu;
+static std::map<int, Engine*> g_engines;
+
 Engine* engine_for_current_device() {
-  static std::mutex mu;
-  static std::map<int, Engine*> engines;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-  std::lock_guard<std::mutex> g(mu);
-  auto it = engines.find(dev);
-  if (it != engines.end()) return it->second;
+  std::lock_guard<std::mutex> g(g_engines_mu);
+  auto it = g_engines.find(dev);
+  if (it != g_engines.end()) return it->second;
   Engine* e = new Engine(dev);
-  engines[dev] = e;
+  g_engines[dev] = e;
   return e;
+}
+
+Engine* existing_engine_for_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  std::lock_guard<std::mutex> g(g_engines_mu);
+  auto it = g_engines.find(dev);
+  return it == g_engines.end() ? nullptr : it->second;
 }
 
 // ---------------------------------------------------------------------------

@@ -226,6 +226,9 @@ class Engine {
   int alloc_outputs(Result* res, int64_t edges, bool dist, bool disp = false);
   std::string tune_report();
   void clear_tuning();
+  // the tuned tunables of the request's workload class, if it has been
+  // tuned already (times nothing); else untimed_tuning(req)
+  Tuning tuned_or_untimed(const Request& req);
 
  private:
   int run_with(const Request& req, const Tuning& t, Result** out);
@@ -263,6 +266,19 @@ class Engine {
 };
 
 Engine* engine_for_current_device();
+// the current device's engine if one was created, else null (creates nothing)
+Engine* existing_engine_for_current_device();
+
+// The tunables a request gets without timing anything: the engine defaults
+// (or the request's own), with the small-request work split below
+// kTuneMinAtoms atoms.
+Tuning untimed_tuning(const Request& req);
+
+// Request builder and cutoff check of the C ABI (nl_capi.cpp).
+int capi_sizes_request(const double* pos, size_t n, const int64_t* sizes, size_t n_sys,
+                       const double* cells, const uint8_t* has_cell, const uint8_t* pbc,
+                       Request& req);
+int capi_check_cutoffs(const double* c, size_t k, const char* msg);
 
 // Graph-captured steady state: a fixed batch shape (systems, cells, cutoffs)
 // whose positions live at a fixed device address and change between runs

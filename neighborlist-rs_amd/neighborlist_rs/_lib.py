@@ -145,6 +145,9 @@ _SIGS = {
                                  C.POINTER(C.c_void_p)]),
     "nl_skin_stats": (C.c_int, [C.c_void_p, i64p, i64p, dp]),
     "nl_skin_free": (None, [C.c_void_p]),
+    "nl_search_noalloc": (C.c_int, [C.c_void_p, C.c_size_t, i64p, C.c_size_t, dp, u8p, u8p, dp,
+                                    C.c_size_t, C.c_int, C.c_int, C.c_uint, C.c_void_p,
+                                    C.c_void_p, C.c_int64, i64p, i64p, C.c_void_p]),
     "nl_cell_inverse": (C.c_int, [dp, dp]),
     "nl_cell_wrap": (C.c_int, [dp, u8p, dp, dp]),
     "nl_device_malloc": (C.c_void_p, [C.c_size_t]),
