@@ -411,6 +411,130 @@ def ethanol_calibration(lib, L, O, threads):
                                    "included), as the published times"}
 
 
+def _median_ms(fn, reps, warm=2):
+    for _ in range(warm):
+        fn()
+    tt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        tt.append(time.perf_counter() - t0)
+    return 1e3 * float(np.median(tt))
+
+
+def config_lines(lib, L, O, threads):
+    """The benchmark configurations one by one, and the reference's own
+    published cases, each with: GPU time with positions and lists on the
+    device (host clock per call, median), the device stages alone (hipEvents),
+    numpy in / numpy out through the Python API (host copies included, as
+    the reference's published times), the roofline fraction of the device
+    stages (24 B per atom + 72 B per cell + 28 B per edge against 8 TB/s), and
+    the oracle on `threads` host cores (None: no CPU timing)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from xyzio import read_xyz
+    import neighborlist_rs as nr
+    from neighborlist_rs import workloads as W
+    out = {}
+
+    def single(name, pos, H, pbc, cutoffs, published=None, reps=30):
+        n = pos.shape[0]
+        multi = len(cutoffs) > 1
+        d = DevArray(lib, np.ascontiguousarray(pos))
+        sz = np.array([n], np.int64)
+        H9, has, pb = cell_arrays([H])
+        if H is not None:
+            pb = np.ascontiguousarray(np.array(pbc, np.uint8).reshape(1, 3))
+        cut = np.array(cutoffs, np.float64)
+        edges = [0]
+
+        def dev():
+            edges[0] = run_systems(lib, L, d.ptr, n, sz, H9, has, pb, cut, multi)
+        dev_ms = _median_ms(dev, reps)
+        stage = []
+        for _ in range(5):
+            dev()
+            stage.append(timing(lib)[3])
+        d.free()
+        cell = None if H is None else nr.PyCell(np.asarray(H).tolist(), [bool(x) for x in pbc])
+        if multi:
+            api = lambda: nr.build_neighborlists_multi(cell, pos, list(cutoffs))
+        else:
+            api = lambda: nr.build_neighborlists(cell, pos, float(cutoffs[0]))
+        api_ms = _median_ms(api, reps)
+        dev_stage_ms = float(np.median(stage))
+        alg = 24.0 * n + (72.0 if H is not None else 0.0) + 28.0 * edges[0]
+        row = {"atoms": int(n), "cutoffs": list(cutoffs), "edges": int(edges[0]),
+               "gpu_device_ms": round(dev_ms, 4), "gpu_device_stages_ms": round(dev_stage_ms, 4),
+               "gpu_numpy_ms": round(api_ms, 4),
+               "roofline_frac": round(alg / (dev_stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+        if O is not None:
+            Hc = None if H is None else np.asarray(H)
+            pc = None if H is None else list(pbc)
+            if multi:
+                cpu = lambda: O.search_single_multi(pos, Hc, pc, list(cutoffs), False)
+            else:
+                cpu = lambda: O.search_single(pos, Hc, pc, float(cutoffs[0]))
+            ocpu = _median_ms(cpu, 5, warm=1)
+            row["oracle_ms"] = round(ocpu, 3)
+            row["oracle_threads"] = threads
+            row["gpu_numpy_speedup_vs_oracle"] = round(ocpu / api_ms, 2)
+        if published:
+            row["reference_published"] = published
+        log(f"config {name}: {row}")
+        out[name] = row
+
+    p1, _, rc1 = W.c1_isolated()
+    single("c1_isolated_1000_rc6", p1, None, None, (rc1,),
+           {"ms": 2.97, "threads": 8, "note": "isolated_1000.xyz rc 6 (not the same positions)",
+            "source": "README.md:109-110"})
+    p2, H2, _ = W.c2_ethanol()
+    for rc, pub in ((6.0, {"ms": 5.35, "threads": 8, "note": "ethanol_1000.xyz",
+                           "source": "README.md:122-123"}),
+                    (14.0, {"ms": 37.88, "threads": 8, "note": "ethanol_1000.xyz",
+                            "source": "README.md:165-166"})):
+        single(f"c2_ethanol_1008_rc{rc:g}", p2, H2, [1, 1, 1], (rc,), pub)
+    p3, H3, _ = W.c3_triclinic()
+    single("c3_triclinic_10008_rc14", p3, H3, [1, 1, 1], (14.0,))
+    pe, rows, pbc = read_xyz("ethanol_10000.xyz")
+    He = np.ascontiguousarray(rows.T)
+    single("ethanol_10000_rc14", pe, He, pbc, (14.0,),
+           {"ms_grace_20_threads": 51.67, "ms_epyc_64_threads": 62.54,
+            "ms_grace_1_thread": 247.26,
+            "source": "benchmarks/benchmark_grace.txt:90-91, benchmark_EPYC7R13.txt:91"})
+    single("ethanol_10000_multi_6_14", pe, He, pbc, (6.0, 14.0),
+           {"ms": 313.0, "threads": 8, "note": "multi-cutoff [6, 14] in one pass",
+            "source": "README.md:188-193"})
+
+    # the reference's GNN batch (batch_radius_benchmark.py:109-133,
+    # batch_results.json:2-12): numpy in, numpy out, no cells
+    pg, bg = W.gnn_molecule_batch()
+    res = [None]
+
+    def gnn():
+        res[0] = nr.build_neighborlists_batch(pg, bg, None, 6.0)
+    api_ms = _median_ms(gnn, 50, warm=5)
+    m = int(res[0]["edge_index"].shape[1])
+    sizes = np.bincount(bg).astype(np.int64)
+    d = DevArray(lib, pg)
+    H9, has, pb = cell_arrays([None] * sizes.size)
+    cut = np.array([6.0])
+    dev_ms = _median_ms(lambda: run_systems(lib, L, d.ptr, pg.shape[0], sizes, H9, has, pb, cut,
+                                            False), 50, warm=5)
+    d.free()
+    row = {"workload": "128 isolated molecules, 42 atoms on average (5,376 atoms), rc 6, "
+                       "build_neighborlists_batch(positions, batch, None, 6.0)",
+           "edges": m, "gpu_numpy_ms": round(api_ms, 4), "gpu_device_ms": round(dev_ms, 4),
+           "reference_published": {"ms_8_cpu": 0.24, "ms_1_cpu": 0.26,
+                                   "source": "benchmarks/batch_results.json:2-12"}}
+    if O is not None:
+        ocpu = _median_ms(lambda: O.search_batch(pg, bg, None, 6.0), 20, warm=2)
+        row["oracle_ms"] = round(ocpu, 3)
+        row["oracle_threads"] = threads
+    log(f"config gnn_batch: {row}")
+    out["gnn_molecule_batch_rc6"] = row
+    return out
+
+
 def precision_report(L, pos, H, rc):
     """FP64 |r_ij| emitted with the edges (cell-list path) against a float64
     recomputation from the raw positions, |p_j + H S - p_i|, for the FP32
@@ -477,6 +601,8 @@ def main():
                     help="run only the single-small-system calls (for rocprofv3)")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the timed batch steps (for rocprofv3)")
+    ap.add_argument("--only-configs", action="store_true",
+                    help="run only the per-configuration lines")
     ap.add_argument("--serve-requests", type=int, default=600,
                     help="requests in the serving trace (0: skip serving)")
     ap.add_argument("--serve-slots", type=int, default=3)
@@ -514,6 +640,14 @@ def main():
 
     if args.only_small:
         print(json.dumps({"small_single_systems": small_single_systems(lib, L)}), flush=True)
+        return
+
+    if args.only_configs:
+        O = None
+        if not args.no_cpu:
+            from oracle import oracle as O
+        threads = O.max_threads() if O is not None else None
+        print(json.dumps({"configs": config_lines(lib, L, O, threads)}), flush=True)
         return
 
     if args.only_serving:
@@ -725,6 +859,11 @@ def main():
         calibration = ethanol_calibration(lib, L, O, threads)
         log(f"ethanol_10000 rc 6: {calibration}")
 
+    Oc = None
+    if not args.no_cpu:
+        from oracle import oracle as Oc
+    configs = config_lines(lib, L, Oc, Oc.max_threads() if Oc is not None else None)
+
     out = {
         "metric": "edges/s and systems/s (batched/served vs plain GPU); p50/p99 build latency",
         "value": round(value, 1),
@@ -765,6 +904,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ethanol_10000_rc6_calibration": calibration,
+        "configs": configs,
     }
     dpos.free()
     d3.free()

@@ -10,6 +10,8 @@
                    cutoffs {5, 6, 14}
   c5_serving_trace Poisson arrivals of single systems, N log-uniform in
                    [16, 50000], isolated / orthorhombic / triclinic, rc U[4, 12]
+  gnn_molecule_batch  the reference's GNN batch: 128 isolated molecules,
+                   42 atoms on average (5,376 atoms), rc 6
 
 Cells are returned as H (lattice vectors as columns), the convention of
 PyCell and of the C ABI.
@@ -143,6 +145,35 @@ def c5_serving_trace(seed=5, n_requests=200, qps=200.0, nmin=16, nmax=50000):
         p, H = random_system(rng, n, kind)
         out.append((float(t[r]), np.ascontiguousarray(p), H, float(rng.uniform(4.0, 12.0))))
     return out
+
+
+def benzene_unit():
+    """12-atom C6H6 ring: C-C 1.39 A, C-H 1.09 A, planar, centred."""
+    ang = np.arange(6) * (np.pi / 3.0)
+    ring = np.stack([np.cos(ang), np.sin(ang), np.zeros(6)], -1)
+    return np.concatenate([1.39 * ring, (1.39 + 1.09) * ring])
+
+
+def gnn_molecule_batch(seed=6, n_molecules=128):
+    """The molecular batch of the reference's GNN benchmark
+    (benchmarks/batch_radius_benchmark.py:15-51): isolated molecules of 30 to
+    50 atoms, each a small molecule replicated along x at a 10 A period
+    (ASE's m * (2, 1, 1) in a 10 A cell) until it has at least 30 atoms,
+    128 molecules averaging 42 atoms (5,376 in all, batch_results.json:2-4).
+    Without ASE: alternately four copies of the ethanol-like unit (36 atoms)
+    and of a benzene ring (48 atoms), each molecule randomly rotated.
+    Returns positions (N, 3) and the int32 batch ids."""
+    rng = np.random.default_rng(seed)
+    units = [ethanol_unit(), benzene_unit()]
+    R = _rotations(rng, n_molecules)
+    pos, ids = [], []
+    for m in range(n_molecules):
+        u = units[m % 2]
+        mol = np.concatenate([u + np.array([10.0 * c, 0.0, 0.0]) for c in range(4)])
+        mol = mol @ R[m].T + rng.uniform(-5.0, 5.0, size=3)
+        pos.append(mol)
+        ids.append(np.full(mol.shape[0], m, np.int32))
+    return np.ascontiguousarray(np.concatenate(pos)), np.concatenate(ids)
 
 
 def batch_arrays(sizes, cells):
