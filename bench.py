@@ -603,6 +603,8 @@ def main():
                     help="run only the timed batch steps (for rocprofv3)")
     ap.add_argument("--only-configs", action="store_true",
                     help="run only the per-configuration lines")
+    ap.add_argument("--bin-order", type=int, default=-1,
+                    help="1: Morton-ranked bins, 0: linear bin order (the engine default)")
     ap.add_argument("--serve-requests", type=int, default=600,
                     help="requests in the serving trace (0: skip serving)")
     ap.add_argument("--serve-slots", type=int, default=3)
@@ -621,6 +623,8 @@ def main():
     from neighborlist_rs import workloads as W
     lib = L.lib
     lib.nl_set_timing(1)
+    if args.bin_order >= 0:
+        lib.nl_set_bin_order(args.bin_order)
     if args.tuning != "auto":
         dv, g, pp, f32 = (int(x) for x in args.tuning.split(","))
         lib.nl_set_autotune(0)

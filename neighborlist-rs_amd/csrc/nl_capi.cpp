@@ -551,6 +551,9 @@ void nl_set_small_path(int mode) {
   nl::config().small_path.store(mode < 0 ? 0 : (mode > 3 ? 3 : mode));
 }
 int nl_get_small_path(void) { return nl::config().small_path.load(); }
+
+void nl_set_bin_order(int morton) { nl::config().bin_order = morton ? 1 : 0; }
+int nl_get_bin_order(void) { return nl::config().bin_order.load(); }
 void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }
 int nl_get_fp32_filter(void) { return nl::config().use_fp32.load(); }
 void nl_set_work_split(int group, int parts) {

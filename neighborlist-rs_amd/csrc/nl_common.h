@@ -68,6 +68,7 @@ struct Params {
   int32_t use_fp32;    // FP32 filter + FP64 band recheck on the cell-list path
   int32_t global_index;// add atom_start to emitted indices (batch API)
   int32_t per_sys_rc;  // rc2 holds [n_sys][k] squared cutoffs instead of [k]
+  int32_t bin_order;   // 0: bins in linear order, 1: ranked by Morton code
   double rc_max;       // max cutoff (NaN-ignoring fold, single.rs:106)
 };
 
@@ -128,6 +129,45 @@ NL_HD int bins_per_cutoff(double rc, double n_atoms, double volume) {
     if (q >= kBinAtoms * (double)(cd * cd * cd) * volume) return cd;
   return 1;  // also for a NaN or zero volume
 }
+// Rank of bin (x, y, z) of an nx x ny x nz grid in the Morton order of its
+// coordinates (x in the lowest bit: interleave(x) | interleave(y) << 1 |
+// interleave(z) << 2, as the reference ranks its bins, search.rs:126-146 and
+// :1177-1185), counted without sorting: going down the code bits, each set
+// bit adds the grid cells that share the higher bits and have a 0 there
+// (the per-axis counts of the sub-box multiply).
+NL_HD int32_t morton_rank(int32_t x, int32_t y, int32_t z, int32_t nx, int32_t ny, int32_t nz) {
+  const int32_t v[3] = {x, y, z}, n[3] = {nx, ny, nz};
+  int B = 0;
+  for (int d = 0; d < 3; ++d) {
+    int b = 0;
+    while (b < 31 && ((int64_t)1 << b) < (int64_t)n[d]) ++b;  // bits of n - 1
+    B = b > B ? b : B;
+  }
+  int64_t q[3] = {0, 0, 0};  // bits of each coordinate taken so far
+  int f[3] = {B, B, B};      // bits of each coordinate still free
+  int64_t r = 0;
+  for (int l = B - 1; l >= 0; --l) {
+    for (int d = 2; d >= 0; --d) {  // code bits 3l+2 (z), 3l+1 (y), 3l (x)
+      const int64_t bit = (v[d] >> l) & 1;
+      if (bit) {
+        int64_t prod = 1;
+        for (int e = 0; e < 3; ++e) {
+          const int fe = e == d ? f[e] - 1 : f[e];
+          const int64_t lo = e == d ? (q[d] << f[d]) : (q[e] << f[e]);
+          int64_t c = (int64_t)n[e] - lo;
+          c = c < 0 ? 0 : c;
+          const int64_t span = (int64_t)1 << fe;
+          prod *= c < span ? c : span;
+        }
+        r += prod;
+      }
+      q[d] = (q[d] << 1) | bit;
+      f[d] -= 1;
+    }
+  }
+  return (int32_t)r;
+}
+
 NL_HD double det3(const double* h) {
   return h[0] * (h[4] * h[8] - h[7] * h[5]) - h[1] * (h[3] * h[8] - h[6] * h[5]) +
          h[2] * (h[3] * h[7] - h[6] * h[4]);

@@ -203,6 +203,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   prm.use_fp32 = tune.use_fp32;
   prm.global_index = req.global_index;
   prm.per_sys_rc = per_sys;
+  prm.bin_order = config().bin_order.load() ? 1 : 0;
   prm.rc_max = rc_max;
   return 0;
 }
@@ -224,6 +225,7 @@ std::vector<size_t> Pipeline::buffer_sizes() const {
       8 * (size_t)(K + 1),                                          // bounds
       8 * (size_t)n_items,                                          // work list
       radix_bytes(),  // radix scratch: digit counts, their offsets, bounds
+      prm.bin_order ? 8 * (size_t)bin_total : 0,                    // Morton tables
   };
 }
 
@@ -250,6 +252,8 @@ void Pipeline::assign(Buffers& b, const std::vector<void*>& p) const {
   b.rcnt = (uint32_t*)p[19];
   b.roff = (int64_t*)((char*)p[19] + roff_at());
   b.rbounds = b.roff + ((int64_t)1 << digit_bits) * rtiles + 1;
+  b.bin_rank = prm.bin_order ? (int32_t*)p[20] : nullptr;
+  b.rank_bin = prm.bin_order ? (int32_t*)p[20] + bin_total : nullptr;
 }
 
 size_t Pipeline::pinned_bytes() const { return off_bounds() + 8 * (size_t)(K + 1) + 64; }
@@ -286,6 +290,8 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   // the sorted keys end in A after an even number of radix passes (in B
   // after the segmented sort)
   A.keys = P.seg_sort ? b.keysB : ((P.passes % 2 == 0) ? b.keysA : b.keysB);
+  A.bin_rank = b.bin_rank;
+  A.rank_bin = b.rank_bin;
   A.cell_start = b.cell_start;
   A.spos64 = b.spos64;
   A.spos32 = b.spos32;
@@ -318,7 +324,8 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   // zero block and counts (items without work keep zero counts)
   NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st));
   NL_TRY(launch_plan(b.pos, b.sys, b.band, b.rc2, prm, st));
-  NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.keysA, b.valsA, st));
+  if (b.bin_rank) NL_TRY(launch_morton(b.sys, S, bin_total, b.bin_rank, b.rank_bin, st));
+  NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.bin_rank, b.keysA, b.valsA, st));
   uint32_t *kin = b.keysA, *vin = b.valsA, *kout = b.keysB, *vout = b.valsB;
   if (seg_sort) {
     NL_TRY(launch_sort_systems(kin, vin, kout, vout, b.sys, b.start, S, seg_passes,

@@ -15,6 +15,8 @@ struct PairArgs {
   const int64_t* sys_start;  // n_sys + 1
   const int64_t* item_base;  // n_sys + 1: work items of system s (atoms, then bins)
   const uint32_t* keys;      // sorted bin keys
+  const int32_t* bin_rank;   // Morton order: [bin_base + linear bin] -> rank (null: linear)
+  const int32_t* rank_bin;   // Morton order: [bin_base + rank] -> linear bin
   const int32_t* cell_start; // total_bins + 1
   const Pos64* spos64;       // sorted wrapped positions (FP64), w = local index
   const float4* spos32;      // sorted wrapped positions (FP32), w = local index bits
@@ -74,7 +76,12 @@ hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity
 hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,
                        const Params& p, hipStream_t st);
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
-                      int64_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
+                      int64_t n, const int32_t* bin_rank, uint32_t* keys, uint32_t* vals,
+                      hipStream_t st);
+// Morton ranks of the bins of every cell-list system: bin_rank[bin_base +
+// linear] = rank, rank_bin[bin_base + rank] = linear
+hipError_t launch_morton(const SysDesc* sys, int n_sys, int64_t total_bins, int32_t* bin_rank,
+                         int32_t* rank_bin, hipStream_t st);
 // one stable pass over digit bits [shift, shift + bits), bits <= 12:
 // cnt [2^bits * tiles] u32, off [2^bits * tiles + 1] i64, bounds [2] i64 and
 // a zeroed scan status [radix_tiles(2^bits * tiles)] + ticket

@@ -337,12 +337,40 @@ __global__ __launch_bounds__(1024) void k_plan(const double* __restrict__ pos,
 }
 
 // ---------------------------------------------------------------------------
-// Binning: one thread per atom.  Key = bin_base + linear bin (cell-list
-// systems) or bin_base (brute-force systems keep their input order).
+// Morton order of the bins (search.rs:126-146): one thread per reserved bin;
+// the bins of the grid k_plan chose get their rank and the inverse entry.
+__global__ __launch_bounds__(256) void k_morton(const SysDesc* __restrict__ sys, int n_sys,
+                                                int64_t total_bins, int32_t* __restrict__ bin_rank,
+                                                int32_t* __restrict__ rank_bin) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total_bins) return;
+  int lo = 0, hi = n_sys - 1;  // the system whose reservation holds bin t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sys[mid].bin_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SysDesc* d = sys + lo;
+  if (d->mode != kCellList) return;
+  const int nx = d->nb[0], ny = d->nb[1], nz = d->nb[2];
+  const int64_t l = t - d->bin_base;
+  if (l >= (int64_t)nx * ny * nz) return;
+  const int x = (int)(l % nx), y = (int)((l / nx) % ny), z = (int)(l / ((int64_t)nx * ny));
+  const int32_t r = morton_rank(x, y, z, nx, ny, nz);
+  bin_rank[t] = r;
+  rank_bin[d->bin_base + r] = (int32_t)l;
+}
+
+// Binning: one thread per atom.  Key = bin_base + bin (cell-list systems;
+// the linear bin, or its Morton rank when bin_rank is given) or bin_base
+// (brute-force systems keep their input order).
 __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ pos,
                                              const SysDesc* __restrict__ sys,
                                              const int64_t* __restrict__ sys_start, int n_sys,
-                                             int64_t n, uint32_t* __restrict__ keys,
+                                             int64_t n, const int32_t* __restrict__ bin_rank,
+                                             uint32_t* __restrict__ keys,
                                              uint32_t* __restrict__ vals) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
@@ -360,7 +388,8 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ pos,
       int v = (int)(u * (double)d->nb[k]);  // u >= 0
       b[k] = v < d->nb[k] - 1 ? v : d->nb[k] - 1;
     }
-    key += (uint32_t)(b[0] + d->nb[0] * (b[1] + d->nb[1] * b[2]));
+    const int lin = b[0] + d->nb[0] * (b[1] + d->nb[1] * b[2]);
+    key += (uint32_t)(bin_rank ? bin_rank[d->bin_base + lin] : lin);
   }
   keys[a] = key;
   vals[a] = (uint32_t)a;
@@ -1029,11 +1058,13 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
   const int nbx = d->nb[0], nby = d->nb[1], nbz = d->nb[2];
   const int nsx = d->ns[0], nsy = d->ns[1], nsz = d->ns[2];
   const int64_t bb = d->bin_base;
-  const int lb = (int)((int64_t)A.keys[p] - bb);
-  const int bin_lo = A.cell_start[bb + lb];
+  // the home bin's key (its Morton rank or linear index) and linear index
+  const int kb = (int)((int64_t)A.keys[p] - bb);
+  const int lb = A.rank_bin ? A.rank_bin[bb + kb] : kb;
+  const int bin_lo = A.cell_start[bb + kb];
   // a group is up to G consecutive slots of one bin, aligned to the bin start
   if (((int)p - bin_lo) % A.group != 0) return;
-  const int hs = (int)p, he = min(A.cell_start[bb + lb + 1], (int)p + A.group);
+  const int hs = (int)p, he = min(A.cell_start[bb + kb + 1], (int)p + A.group);
   const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
   const double h0 = d->h[0], h1 = d->h[1], h2 = d->h[2], h3 = d->h[3], h4 = d->h[4],
                h5 = d->h[5], h6 = d->h[6], h7 = d->h[7], h8 = d->h[8];
@@ -1113,7 +1144,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
         const float lb2 = diag ? (gx * gx + gy * gy) + gz * gz
                                : fmaxf(fmaxf(gx * gx, gy * gy), gz * gz);
         if (lb2 <= prune2) {
-          const int64_t bin = bb + cx + (int64_t)nbx * (cy + (int64_t)nby * cz);
+          const int lin = cx + nbx * (cy + nby * cz);
+          const int64_t bin = bb + (A.bin_rank ? A.bin_rank[bb + lin] : lin);
           st = A.cell_start[bin];
           const int en = A.cell_start[bin + 1];
           if (t == tc) st = hb;  // home bin, no image: slots from this group on
@@ -2034,10 +2066,19 @@ hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const doub
 }
 
 hipError_t launch_bin(const double* pos, const SysDesc* sys, const int64_t* sys_start, int n_sys,
-                      int64_t n, uint32_t* keys, uint32_t* vals, hipStream_t st) {
+                      int64_t n, const int32_t* bin_rank, uint32_t* keys, uint32_t* vals,
+                      hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bin, dim3(grid_for(n, 256)), dim3(256), 0, st, pos, sys, sys_start, n_sys,
-                     n, keys, vals);
+                     n, bin_rank, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_morton(const SysDesc* sys, int n_sys, int64_t total_bins, int32_t* bin_rank,
+                         int32_t* rank_bin, hipStream_t st) {
+  if (total_bins <= 0 || n_sys <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton, dim3(grid_for(total_bins, 256)), dim3(256), 0, st, sys, n_sys,
+                     total_bins, bin_rank, rank_bin);
   return hipGetLastError();
 }
 

@@ -96,6 +96,8 @@ _SIGS = {
     "nl_get_cell_div": (C.c_int, []),
     "nl_set_small_path": (None, [C.c_int]),
     "nl_get_small_path": (C.c_int, []),
+    "nl_set_bin_order": (None, [C.c_int]),
+    "nl_get_bin_order": (C.c_int, []),
     "nl_set_fp32_filter": (None, [C.c_int]),
     "nl_get_fp32_filter": (C.c_int, []),
     "nl_set_work_split": (None, [C.c_int, C.c_int]),

@@ -183,6 +183,15 @@ def test_thresholds_and_threads(nr):
     nr.init_logging(None)
 
 
+def test_engine_setters_roundtrip(nr):
+    from neighborlist_rs._lib import lib
+    old = lib.nl_get_bin_order()
+    for v, want in ((1, 1), (0, 0), (7, 1)):
+        lib.nl_set_bin_order(v)
+        assert lib.nl_get_bin_order() == want
+    lib.nl_set_bin_order(old)
+
+
 # ---- torch surface (neighborlist_rs.tensors): validation before device work --
 def test_torch_surface_validation(nr):
     import torch

@@ -282,6 +282,36 @@ def test_hundred_cutoffs_vs_oracle(nr, oracle, disjoint):
         assert_same_edges(as_tuple(got[k]), want[k], f"batch list {k} rc={cut[k]:.3f}")
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_bin_order_linear_and_morton(nr, oracle, order):
+    """Bins keyed by linear index (0) or by Morton rank (1, the default,
+    search.rs:126-146): the same edge sets either way (a 10k-atom triclinic
+    system, the many-image C2 at rc 14, a ragged multi-cutoff batch with
+    disjoint shells)."""
+    from neighborlist_rs import _lib, workloads as W
+    old = _lib.lib.nl_get_bin_order()
+    _lib.lib.nl_set_bin_order(order)
+    try:
+        p3, H3, _ = W.c3_triclinic()
+        _print_case("bin order", order, "c3 rc6")
+        assert_same_edges(_gpu_single(nr, p3, H3, [True] * 3, 6.0),
+                          oracle.search_single(p3, H3, [1, 1, 1], 6.0), f"order {order} c3")
+        p2, H2, _ = W.c2_ethanol()
+        _print_case("bin order", order, "c2 rc14")
+        assert_same_edges(_gpu_single(nr, p2, H2, [True] * 3, 14.0),
+                          oracle.search_single(p2, H2, [1, 1, 1], 14.0), f"order {order} c2")
+        p, sizes, cells, cut = W.c4_ragged_batch(n_systems=24, nmax=2048, seed=77)
+        batch, rows = W.batch_arrays(sizes, cells)
+        _print_case("bin order", order, "ragged batch")
+        ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+        want = oracle.search_batch_multi(p, batch, ocells, list(cut), True)
+        got = nr.build_neighborlists_batch_multi(p, batch, rows, list(cut), disjoint=True)
+        for k in range(len(cut)):
+            assert_same_edges(as_tuple(got[k]), want[k], f"order {order} batch list {k}")
+    finally:
+        _lib.lib.nl_set_bin_order(old)
+
+
 def test_parallel_flag_does_not_change_results(nr, oracle):
     """parallel=False selects the reference's serial search (search.rs:275-289);
     the edge set is the same."""
