@@ -924,10 +924,23 @@ Tuning Engine::tuning_for(const Request& req) {
       x.parts = p;
       cand.emplace_back(time_one(x, 3), x);
     }
-  std::sort(cand.begin(), cand.end(),
-            [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) {
-              return a.first < b.first;
-            });
+  auto by_time = [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) {
+    return a.first < b.first;
+  };
+  std::sort(cand.begin(), cand.end(), by_time);
+  // ... the bin size again under the two best work splits (the first pass
+  // chose it under the default split; the best bin size depends on the
+  // split) ...
+  for (int c = 0; c < 2; ++c) {
+    const Tuning base = cand[c].second;
+    for (int dv : {0, 1, 2, 3, 4}) {
+      if (dv == base.cell_div) continue;
+      Tuning x = base;
+      x.cell_div = dv;
+      cand.emplace_back(time_one(x, 3), x);
+    }
+  }
+  std::sort(cand.begin(), cand.end(), by_time);
   {
     Tuning x = cand[0].second;
     x.use_fp32 = 1 - x.use_fp32;
