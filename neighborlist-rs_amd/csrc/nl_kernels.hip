@@ -734,18 +734,24 @@ __global__ __launch_bounds__(256) void k_finalize(
 // Pair search.  One wavefront per unit; a block holds kWaves units.
 namespace {
 
+// Home atoms staged per round of the home-atom loop.
+constexpr int kHomeAtoms = 32;
+
+// One wave's LDS: 4.1 KB in the FP64 fill pass, 3.6 KB in the FP64 count
+// pass (so registers, not LDS, bound the waves per CU).
+template <bool F32, bool FILL>
 struct WaveLds {
-  int32_t bs[3][64];  // bin-image shift of the segment
-  double off[3][64];  // H bs in FP64, reference order
-  float off32[3][64]; // H bs in FP32 (filter)
+  int32_t bs[3][64];                  // bin-image shift of the segment
+  double off[3][64];                  // H bs in FP64, reference order
+  float off32[F32 ? 3 : 1][F32 ? 64 : 1];  // H bs in FP32 (filter)
   // home atoms of the item, read at one uniform address per step (a
   // broadcast LDS read instead of a readlane per coordinate)
   union {
-    Pos64 home64[64];   // FP64 wrapped position, w = local index (FP64 kernels)
-    float4 home32[64];  // FP32 wrapped position, w = local index bits (FP32 kernels)
+    Pos64 home64[kHomeAtoms];   // FP64 wrapped position, w = local index (FP64 kernels)
+    float4 home32[kHomeAtoms];  // FP32 wrapped position, w = local index bits (FP32 kernels)
   };
-  int4 homesi[64];      // wrap shifts, w = local index (fill pass)
-  int32_t mark[64];     // candidate lane -> segment starting there (-1: none)
+  int4 homesi[FILL ? kHomeAtoms : 1];      // wrap shifts, w = local index (fill pass)
+  int32_t mark[64];                   // candidate lane -> segment starting there (-1: none)
 };
 
 // q = v / m, r = v % m for 0 <= v < 2^22, 1 <= m < 2^22: a float estimate
@@ -948,12 +954,12 @@ __device__ __forceinline__ void put(Out<NQ, DIST, DISP>& O, int g, uint64_t m, u
   O.c[g] += (FULL ? 2u : 1u) * (uint32_t)__popcll(m);
 }
 
+
 // Brute-force row i of a system (one image per pair, i < j in row-major
 // order, search.rs:1187-1405).
-template <bool FILL, int NQ, bool FULL, bool DIST, bool DISP>
+template <bool FILL, int NQ, bool FULL, bool DIST, bool DISP, class SinkT>
 __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ>& C,
-                                                const SysDesc* d, int64_t u, int mode,
-                                                Sink<FILL, NQ, DIST, DISP>& S) {
+                                                const SysDesc* d, int64_t u, int mode, SinkT& S) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
   const int n = d->n_atoms;
@@ -1013,7 +1019,7 @@ __device__ __forceinline__ void unit_bruteforce(const PairArgs& A, const Cuts<NQ
       const uint32_t gj = (uint32_t)(A.global_index ? a0 + j : (int64_t)j);
 #pragma unroll
       for (int g = 0; g < NQ; ++g)
-        put<NQ, FULL, DIST, DISP>(S, g, m[g], gi, gj, sx, sy, sz, d2, dx, dy, dz);
+        put<NQ, FULL>(S, g, m[g], gi, gj, sx, sy, sz, d2, dx, dy, dz);
     } else {
 #pragma unroll
       for (int g = 0; g < NQ; ++g) S.c[g] += (uint32_t)__popcll(m[g]);
@@ -1049,9 +1055,9 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 // Kinds of candidate chunk (see the home-atom loop of unit_cell).
 constexpr int kPlain = 0, kSelf = 1, kHome = 2;
 
-template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
+template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP, class Lds>
 __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, const SysDesc* d,
-                                          int64_t p, int part, WaveLds& L,
+                                          int64_t p, int part, Lds& L,
                                           Sink<FILL, NQ, DIST, DISP>& S) {
   const int lane = lane_id();
   const int64_t a0 = d->atom_start;
@@ -1092,14 +1098,14 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                     d->h[6] == 0.0 && d->h[7] == 0.0;
   const float prune2 = (float)(d->rc_max * d->rc_max * (1.0 + 1e-4));
   const int64_t base_idx = A.global_index ? a0 : 0;
-  for (int hb = hs; hb < he; hb += 64) {
-    const int hn = uniform_i32(min(64, he - hb));
+  for (int hb = hs; hb < he; hb += kHomeAtoms) {
+    const int hn = uniform_i32(min(kHomeAtoms, he - hb));
     if (lane < hn) {
       if (F32)
         L.home32[lane] = A.spos32[hb + lane];
       else
         L.home64[lane] = A.spos64[hb + lane];
-      if (FILL) L.homesi[lane] = A.sidx[hb + lane];
+      if constexpr (FILL) L.homesi[lane] = A.sidx[hb + lane];
     }
     for (int t0 = t_lo; t0 < t_hi; t0 += 64) {
       const int t = t0 + lane;
@@ -1165,7 +1171,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
       L.off[0][lane] = ox;
       L.off[1][lane] = oy;
       L.off[2][lane] = oz;
-      if (F32) {
+      if constexpr (F32) {
         const float g0 = d->h32[0], g1 = d->h32[1], g2 = d->h32[2], g3 = d->h32[3],
                     g4 = d->h32[4], g5 = d->h32[5], g6 = d->h32[6], g7 = d->h32[7],
                     g8 = d->h32[8];
@@ -1234,12 +1240,12 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
           }
           if constexpr (FILL) S.rebase();
           const bool home_seg = (t0 == tc) && (csg == 0);
-          const double cox = L.off[0][csg], coy = L.off[1][csg], coz = L.off[2][csg];
           const int sbx = L.bs[0][csg], sby = L.bs[1][csg], sbz = L.bs[2][csg];
+          const double cox = L.off[0][csg], coy = L.off[1][csg], coz = L.off[2][csg];
           // FP32 filter coordinates: candidate plus image offset, once per
           // candidate (the band covers this order of rounding as well)
           float px = 0.f, py = 0.f, pz = 0.f;
-          if (F32) {
+          if constexpr (F32) {
             px = c32.x + L.off32[0][csg];
             py = c32.y + L.off32[1][csg];
             pz = c32.z + L.off32[2][csg];
@@ -1350,15 +1356,15 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
                 }
 #pragma unroll
                 for (int g = 0; g < NQ; ++g)
-                  put<NQ, FULL, DIST, DISP>(S, g, m[g], gi, gj, sx, sy, sz, d2, ex, ey, ez);
+                  put<NQ, FULL>(S, g, m[g], gi, gj, sx, sy, sz, d2, ex, ey, ez);
                 // the self image's -S copy (full lists get it from the mirror)
                 if (KIND != kPlain && !FULL) {
                   const uint64_t self = __ballot(oj == oi) & any;
                   if (self) {
 #pragma unroll
                     for (int g = 0; g < NQ; ++g)
-                      put<NQ, false, DIST, DISP>(S, g, m[g] & self, gi, gj, -sx, -sy, -sz, d2,
-                                                 -ex, -ey, -ez);
+                      put<NQ, false>(S, g, m[g] & self, gi, gj, -sx, -sy, -sz, d2, -ex, -ey,
+                                     -ez);
                   }
                 }
               }
@@ -1398,8 +1404,8 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
 // takes the cell-list path; slots that do not start a group are empty).
 // Edges of one system are thus contiguous and systems come out in input
 // order, as batch.rs:135-144 concatenates them.
-template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
-__device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, WaveLds& L) {
+template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP, class Lds>
+__device__ __forceinline__ void pair_item(const PairArgs& A, int64_t u, Lds& L) {
   const int s = uniform_i32(find_sys(A.item_base, A.n_sys, u));
   const SysDesc* d = A.sys + s;
   const int mode = uniform_i32(d->mode);
@@ -1522,7 +1528,8 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 // part; the others leave at once.
 template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
 __device__ __forceinline__ void pairs_body(const PairArgs& A) {
-  __shared__ WaveLds s_lds[kWaves];
+  using Lds = WaveLds<F32, FILL>;
+  __shared__ Lds s_lds[kWaves];
   const int w = uniform_i32(threadIdx.x >> 6);
   const int64_t front = A.n_real[0];
   const int64_t total = front + (int64_t)A.n_real[1];
