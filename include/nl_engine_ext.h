@@ -102,8 +102,13 @@ int nl_get_cell_div(void);
  * and for cells whose widths leave twice the largest cutoff clear of the
  * minimum-image bound; 2: one launch, 3: two launches, for every single
  * system of at most 1024 atoms (the kernels' plan hands cell-list systems
- * back to the pipeline); 0: off.  No counterpart in the reference (a GPU
- * launch choice). */
+ * back to the pipeline); 0: off.  Batches of two or more systems, each of at
+ * most 1024 atoms and below the brute-force threshold (at most 65,536
+ * atoms x lists in all, no per-system ranges), take three launches whenever
+ * the mode is not 0: counts (one row per wave, four rows per block, every
+ * block planning its own system), a one-block scan, the fill; modes 2 and 3
+ * skip the host's minimum-image screen there too.  No counterpart in the
+ * reference (a GPU launch choice). */
 void nl_set_small_path(int mode);
 int nl_get_small_path(void);
 /* Order of the cell-list bins, and so of the sorted atoms and the work items:

@@ -924,7 +924,7 @@ int nl_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hip
 // Pinned host blocks for lists delivered to host arrays: a device-to-host
 // copy into page-locked memory runs at the link's DMA rate, into pageable
 // memory through the runtime's staging buffer at a fraction of it.  Blocks
-// come in power-of-two sizes from 1 MiB; freed blocks are kept for reuse up
+// come in power-of-two sizes from 64 KiB; freed blocks are kept for reuse up
 // to kPinnedCache bytes, and at most kPinnedLive bytes are handed out at once
 // (beyond it nl_host_alloc returns NULL and callers use pageable memory).
 namespace {
@@ -944,7 +944,7 @@ PinnedPool& pinned_pool() {
 
 void* nl_host_alloc(size_t bytes) {
   if (bytes == 0) return nullptr;
-  size_t cls = size_t(1) << 20;
+  size_t cls = size_t(1) << 16;
   while (cls < bytes) cls <<= 1;
   PinnedPool& P = pinned_pool();
   std::lock_guard<std::mutex> g(P.mu);

@@ -380,7 +380,7 @@ hipError_t Pipeline::enqueue_bounds(const Buffers& b, char* pin, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Engine
 
-enum { kRangesSlot = kNumBuffers, kSlots };
+enum { kRangesSlot = kNumBuffers, kSmallBatchSlot, kSlots };
 
 Engine::Engine(int device) : device_(device), scratch_(kSlots) {
   (void)hipSetDevice(device_);
@@ -536,6 +536,12 @@ int Engine::run_with(const Request& req, const Tuning& t, Result** out) {
       return r;
     }
     // the plan took the cell-list path: the pipeline below decides alike
+  } else if (small_batch_eligible(req, P)) {
+    const int r = run_small_batch(req, P, res.get());
+    if (r != 2) {
+      if (r == 0) *out = res.release();
+      return r;
+    }
   }
   const int S = P.S;
   std::vector<size_t> sz = P.buffer_sizes();
@@ -684,12 +690,24 @@ enum : size_t {
   kSmBounds = 0,
   kSmBand = 128,
   kSmTicket = 192,
+  kSmBatchFlag = 224,  // cell-list flag of the batched form
   kSmPlanned = 256,
   kSmCounts = 768,
   kSmOffs = kSmCounts + 4 * (size_t)kGroup * kSmallRows,
   kSmBytes = kSmOffs + 4 * (size_t)kGroup * kSmallRows,
 };
 static_assert(sizeof(SysDesc) <= kSmCounts - kSmPlanned, "descriptor slot");
+
+int Engine::ensure_small_buf() {
+  if (small_buf_) return 0;
+  if (hipMalloc(&small_buf_, kSmBytes) != hipSuccess) {
+    small_buf_ = nullptr;
+    set_error("out of device memory");
+    return 1;
+  }
+  NL_CHECK(hipMemset(small_buf_, 0, kSmBytes));  // the tickets start at zero
+  return 0;
+}
 
 // 0: done, 1: error, 2: the plan took the cell-list path (nothing written)
 int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
@@ -705,14 +723,7 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
       return 1;
     }
   }
-  if (!small_buf_) {
-    if (hipMalloc(&small_buf_, kSmBytes) != hipSuccess) {
-      small_buf_ = nullptr;
-      set_error("out of device memory");
-      return 1;
-    }
-    NL_CHECK(hipMemset(small_buf_, 0, kSmBytes));  // the ticket starts at zero
-  }
+  if (ensure_small_buf()) return 1;
   // page-locked block: the bounds read back, then the host positions (at
   // most 24 KB: one memcpy and an asynchronous upload instead of the
   // runtime's staged copy of pageable memory)
@@ -819,6 +830,165 @@ int Engine::run_small(const Request& req, const Pipeline& P, Result* res) {
     times_ = StageTimes{};
     times_.count = times_.total = tm;
     times_.units = P.n;
+    times_.edges = total;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of small systems that all take a brute-force path, without the
+// pipeline: three launches (count, a one-block scan, fill) instead of the
+// pipeline's dozen, whose binning, sorting and work-list stages do nothing
+// for brute-force systems.  The host screens as for single
+// systems (every system below the brute-force threshold and at most
+// kSmallRows atoms; with a cell, widths that leave the cutoff clear of the
+// minimum-image bound); the plan inside the kernels decides, and a batch in
+// which some system takes the cell-list path goes to the pipeline.  One block
+// scans the [list][atom] counts, so the batch is limited to kBatchSmallRows
+// of them.
+constexpr int64_t kBatchSmallRows = int64_t(1) << 16;
+
+bool Engine::small_batch_eligible(const Request& req, const Pipeline& P) const {
+  const int on = config().small_path.load();
+  if (on == 0 || P.S < 2 || P.n < 1 || P.K > kGroup || req.want_ranges || P.per_sys ||
+      (int64_t)P.K * P.n > kBatchSmallRows)
+    return false;
+  const uint64_t thr = config().bf_threshold.load();
+  int64_t most = 0;
+  for (const SysDesc& d : P.desc) {
+    if (d.n_atoms > kSmallRows || (uint64_t)d.n_atoms >= thr) return false;
+    if (d.has_cell && on < 2) {  // modes 2 and 3 leave the screen to the plan (tests)
+      double inv[9], w[3];
+      if (!inverse3(d.h, inv)) return false;
+      widths3(inv, w);
+      if (!(d.rc_max * 2.0 < 0.999 * std::fmin(std::fmin(w[0], w[1]), w[2]))) return false;
+    }
+    most += (int64_t)d.n_atoms * (d.n_atoms - 1) / 2;
+  }
+  // offsets are 32-bit: never more than the brute force can find
+  return most * P.K * (req.full ? 2 : 1) < (int64_t(1) << 31);
+}
+
+// 0: done, 1: error, 2: some system took the cell-list path (nothing written)
+int Engine::run_small_batch(const Request& req, const Pipeline& P, Result* res) {
+  const int K = P.K, S = P.S;
+  const int64_t n = P.n;
+  // block table: first block of each system, then the system of each block
+  int64_t most = 0, blocks64 = 0;
+  for (int s = 0; s < S; ++s) {
+    const int64_t ns = P.desc[s].n_atoms;
+    blocks64 += (ns + kBatchRows - 1) / kBatchRows;
+    most += ns * (ns - 1) / 2;
+  }
+  const int blocks = (int)blocks64;
+  std::vector<int32_t> blk((size_t)S + 1 + (size_t)blocks, 0);
+  for (int s = 0; s < S; ++s) {
+    const int32_t nb = (int32_t)((P.desc[s].n_atoms + kBatchRows - 1) / kBatchRows);
+    blk[s + 1] = blk[s] + nb;
+    for (int32_t b = blk[s]; b < blk[s + 1]; ++b) blk[(size_t)S + 1 + b] = s;
+  }
+  most = std::max<int64_t>(1, most * K * (req.full ? 2 : 1));
+  // device block: descriptors and block table (one upload), planned
+  // descriptors, bands, counts, offsets, bounds
+  auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+  const size_t o_blk = sizeof(SysDesc) * S;
+  const size_t blk_bytes = 4 * blk.size();
+  const size_t o_planned = up(o_blk + blk_bytes);
+  const size_t o_band = up(o_planned + sizeof(SysDesc) * S);
+  const size_t o_counts = up(o_band + sizeof(float2) * (size_t)S * K);
+  const size_t o_offs = up(o_counts + 4 * (size_t)K * n);
+  const size_t o_bounds = up(o_offs + 4 * (size_t)K * n);
+  const size_t dev_bytes = o_bounds + 8 * ((size_t)K + 1);
+  char* dv = (char*)scratch(kSmallBatchSlot, dev_bytes);
+  double* dpos = const_cast<double*>(req.pos);
+  if (!P.pos_device) dpos = (double*)scratch(0, sizeof(double) * 3 * (size_t)n);
+  if (!dv || !dpos) {
+    set_error("out of device memory");
+    return 1;
+  }
+  if (ensure_small_buf()) return 1;
+  // page-locked block: the bounds read back, the upload, the host positions
+  constexpr size_t kPinUp = 8 * (kGroup + 1) + 56;
+  const size_t up_bytes = o_blk + blk_bytes;
+  const size_t o_pinpos = up(kPinUp + up_bytes);
+  const size_t pos_bytes = sizeof(double) * 3 * (size_t)n;
+  char* pin = (char*)pinned(o_pinpos + (P.pos_device ? 0 : pos_bytes));
+  if (!pin) {
+    set_error("out of pinned host memory");
+    return 1;
+  }
+  int64_t* hb = (int64_t*)pin;
+  std::memcpy(pin + kPinUp, P.desc.data(), sizeof(SysDesc) * S);
+  std::memcpy(pin + kPinUp + o_blk, blk.data(), blk_bytes);
+
+  BatchSmallArgs a{};
+  a.pos = dpos;
+  a.sys = (const SysDesc*)dv;
+  a.blk_start = (const int32_t*)(dv + o_blk);
+  a.blk_sys = a.blk_start + S + 1;
+  a.n_sys = S;
+  a.n = n;
+  for (int q = 0; q < K; ++q) a.rc2[q] = P.rc2[q];
+  a.prm = P.prm;
+  a.band = (float2*)(dv + o_band);
+  a.bounds = (int64_t*)(dv + o_bounds);
+  a.k = K;
+  a.disjoint = P.disjoint;
+  a.full = P.full;
+  a.counts = (uint32_t*)(dv + o_counts);
+  a.offs = (uint32_t*)(dv + o_offs);
+  a.planned = (SysDesc*)(dv + o_planned);
+  a.flag = (uint32_t*)((char*)small_buf_ + kSmBatchFlag);
+
+  hipStream_t st = stream_;
+  if (req.has_ustream) {
+    NL_CHECK(hipEventRecord(ev_[5], req.ustream));
+    NL_CHECK(hipStreamWaitEvent(st, ev_[5], 0));
+  }
+  if (!P.pos_device) {
+    std::memcpy(pin + o_pinpos, req.pos, pos_bytes);
+    NL_CHECK(hipMemcpyAsync(dpos, pin + o_pinpos, pos_bytes, hipMemcpyHostToDevice, st));
+  }
+  NL_CHECK(hipMemcpyAsync(dv, pin + kPinUp, up_bytes, hipMemcpyHostToDevice, st));
+  // room from the last request of the same shape (the pipeline's signature:
+  // both paths give the same lists), else a small default
+  const int64_t sig[4] = {P.n, S, K, (int64_t)req.full * 4 + (int64_t)req.multi * 2 + req.disjoint};
+  const bool have_est = last_total_ >= 0 && std::equal(sig, sig + 4, last_sig_);
+  const int64_t cap = std::min(most, (have_est ? last_total_ + last_total_ / 8 : 0) + 4096);
+  auto launch = [&](int64_t room, bool count) -> int {
+    if (alloc_outputs(res, room, req.want_dist, req.want_disp)) return 1;
+    a.edges = (int64_t*)res->edges;
+    a.shifts = (int32_t*)res->shifts;
+    a.dist = (double*)res->dist;
+    a.disp = (double*)res->disp;
+    a.capacity = room;
+    if (timing_) NL_CHECK(hipEventRecord(ev_[0], st));
+    if (count) NL_CHECK(launch_batch_brute(false, a, blocks, st));
+    NL_CHECK(launch_batch_brute(true, a, blocks, st));
+    if (a.dist) NL_CHECK(launch_sqrt_dist(a.dist, a.bounds + K, room, st));
+    if (timing_) NL_CHECK(hipEventRecord(ev_[4], st));
+    NL_CHECK(hipMemcpyAsync(hb, a.bounds, 8 * (K + 1), hipMemcpyDeviceToHost, st));
+    NL_CHECK(hipStreamSynchronize(st));
+    return 0;
+  };
+  if (launch(cap, true)) return 1;
+  if (hb[K] < 0) return 2;
+  const int64_t total = hb[K];
+  if (total > cap && launch(total, false)) return 1;
+  res->total = total;
+  for (int q = 0; q < K; ++q) {
+    const int r = P.rank_of(q);
+    res->base[q] = hb[r];
+    res->len[q] = hb[r + 1] - hb[r];
+  }
+  for (int q = 0; q < 4; ++q) last_sig_[q] = sig[q];
+  last_total_ = total;
+  if (timing_) {
+    float tm = 0;
+    (void)hipEventElapsedTime(&tm, ev_[0], ev_[4]);
+    times_ = StageTimes{};
+    times_.count = times_.total = tm;
+    times_.units = n;
     times_.edges = total;
   }
   return 0;

@@ -240,6 +240,9 @@ class Engine {
   int run_with(const Request& req, const Tuning& t, Result** out);
   bool small_eligible(const Request& req, const Pipeline& P) const;
   int run_small(const Request& req, const Pipeline& P, Result* res);
+  bool small_batch_eligible(const Request& req, const Pipeline& P) const;
+  int run_small_batch(const Request& req, const Pipeline& P, Result* res);
+  int ensure_small_buf();
   void* scratch(int slot, size_t bytes);
   void* pinned(size_t bytes);
 

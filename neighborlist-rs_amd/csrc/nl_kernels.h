@@ -65,6 +65,37 @@ struct SmallArgs {
   uint32_t* ticket;     // zero before the count launch; the last block re-arms it
 };
 
+// A batch of small systems that all take a brute-force path (each at most
+// kSmallRows atoms): ceil(n_s / kBatchRows) blocks of kBatchRows rows (one
+// wave each) per system.
+constexpr int kBatchRows = 4;
+struct BatchSmallArgs {
+  const double* pos;          // raw positions of the batch (n, 3), f64
+  const SysDesc* sys;         // [n_sys] descriptors before planning
+  const int32_t* blk_start;   // [n_sys + 1] first block of each system
+  const int32_t* blk_sys;     // [blocks] system of each block
+  int32_t n_sys;
+  int64_t n;                  // atoms in the batch
+  double rc2[kGroup];         // k squared cutoffs, ascending
+  Params prm;
+  float2* band;               // [n_sys][k], written by the plan (cell-list path only)
+  int64_t* bounds;            // [k + 1]; bounds[k] = -1 when some system takes the cell list
+  int64_t* edges;             // outputs laid out as for the pair kernels
+  int32_t* shifts;
+  double* dist;
+  double* disp;
+  int64_t capacity;           // edges the outputs hold; a larger total writes only bounds
+  int32_t k, disjoint, full;
+  uint32_t* counts;           // [k][n] per-row counts
+  uint32_t* offs;             // [k][n] per-row offsets over all lists (total < 2^32)
+  SysDesc* planned;           // [n_sys] descriptors after planning (count -> fill)
+  uint32_t* flag;             // some system took the cell-list path: zero before the
+                              // count launch, re-armed by the scan
+};
+// count and scan launches (fill = false; writes bounds), then the fill
+// launch (fill = true)
+hipError_t launch_batch_brute(bool fill, const BatchSmallArgs& a, int blocks, hipStream_t st);
+
 // one block (k_small)
 hipError_t launch_small(const SmallArgs& a, hipStream_t st);
 // count launch (fill = false; writes bounds), then fill launch (fill = true)

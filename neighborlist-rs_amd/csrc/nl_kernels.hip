@@ -159,8 +159,8 @@ __global__ void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restri
 // the reference's edge set.
 //
 // plan_system: the plan of system `d`, called by every thread of a block of
-// 1024 (k_plan, and k_small for a single small system); `band` and `rc2s`
-// are this system's [k] thresholds and squared cutoffs.  Thread 0 writes the
+// up to 1024 (k_plan and the small-system kernels); `band` and `rc2s` are
+// this system's [k] thresholds and squared cutoffs.  Thread 0 writes the
 // descriptor; the caller synchronises the block before reading it.
 __device__ __forceinline__ void plan_system(const double* __restrict__ pos, SysDesc* d,
                                             float2* __restrict__ band,
@@ -213,10 +213,11 @@ __device__ __forceinline__ void plan_system(const double* __restrict__ pos, SysD
   double occupied = 1.0;  // volume the atoms occupy (density for the bin size)
   if (!d->has_cell) {
     const double margin = rc_max + 1.0;  // AUTO_BOX_MARGIN (single.rs:6)
+    const int waves = (int)(blockDim.x >> 6);
     for (int k = 0; k < 9; ++k) h[k] = 0.0;
     for (int k = 0; k < 3; ++k) {
       double mn = s_mn[k][0], mx = s_mx[k][0];
-      for (int w = 1; w < kWavesPlan; ++w) {
+      for (int w = 1; w < waves; ++w) {
         mn = s_mn[k][w] < mn ? s_mn[k][w] : mn;
         mx = s_mx[k][w] > mx ? s_mx[k][w] : mx;
       }
@@ -1816,6 +1817,132 @@ __global__ __launch_bounds__(1024) void k_brute_fill(SmallArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Batches of small systems that all take a brute-force path, without the
+// pipeline: the count launch gives each system ceil(n_s / kBatchRows) blocks
+// of kBatchRows rows (one row per wave); every block plans its own system
+// (plan_system on the same inputs: the same plan) and counts its rows into
+// [list][atom].  One block then scans all counts, and the fill launch writes
+// the rows at the scanned offsets.  Rows, their order and their offsets are
+// the pipeline's (per list: systems in order, rows in order; cell-list items
+// of brute-force systems count zero there), so the lists are byte-identical
+// to it.  A system whose plan takes the cell-list path sets a flag; the scan
+// then writes bounds[k] = -1 and the fill writes nothing.  No block waits for
+// another: with a last-block ticket instead of the scan launch (each block
+// fencing its counts out of its XCD's L2 before taking the ticket) the count
+// launch of 384 blocks took 118 us, without it 21 us.  Blocks are small
+// because each pays the plan's latency once: 16-wave blocks fit one per CU
+// at these registers, so 384 of them ran in two rounds.
+
+template <int NQ, bool FULL>
+__global__ __launch_bounds__(64 * kBatchRows) void k_batch_brute_count(BatchSmallArgs a) {
+  __shared__ SysDesc s_d;
+  __shared__ double s_rc2[kGroup];
+  const int tid = threadIdx.x;
+  const int b = (int)blockIdx.x;
+  const int s = uniform_i32(a.blk_sys[b]);
+  if (tid == 0) {
+    s_d = a.sys[s];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) s_rc2[g] = a.rc2[g];
+  }
+  __syncthreads();
+  plan_system(a.pos, &s_d, a.band + (int64_t)s * a.k, s_rc2, a.prm);
+  __syncthreads();
+  const int mode = s_d.mode;
+  if (mode == kCellList) {
+    if (tid == 0) atomicOr(a.flag, 1u);
+    return;
+  }
+  const int64_t a0 = s_d.atom_start;
+  const int i = (b - a.blk_start[s]) * kBatchRows + uniform_i32(tid >> 6);
+  if (b == a.blk_start[s] && tid == 0) a.planned[s] = s_d;
+  if (i >= s_d.n_atoms) return;
+  PairArgs A{};
+  A.pos = a.pos;
+  A.global_index = a.prm.global_index;
+  const Cuts<NQ> C = small_cuts<NQ>(s_rc2, a.k, a.disjoint);
+  Counts<NQ> cnt;
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) cnt.c[g] = 0;
+  unit_bruteforce<false, NQ, FULL, false, false>(A, C, &s_d, a0 + i, mode, cnt);
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    const uint32_t t = FULL ? 2u * cnt.c[g] : cnt.c[g];  // wave-uniform
+    if (g < a.k && lane_id() == 0) a.counts[(int64_t)g * a.n + a0 + i] = t;
+  }
+}
+
+// One block: exclusive scan of the [list][atom] counts, kChunk entries at a
+// time, into offsets and list bounds (or bounds[k] = -1 when a system took
+// the cell-list path; the flag is re-armed for the next call).
+__global__ __launch_bounds__(1024) void k_batch_scan(BatchSmallArgs a) {
+  constexpr int kChunk = kGroup * kSmallRows;  // entries small_scan takes at once
+  __shared__ uint32_t s_off[kChunk];
+  __shared__ uint32_t s_wsum[16];
+  __shared__ int64_t s_b[kGroup];
+  __shared__ uint32_t s_flag;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_flag = *a.flag;
+  __syncthreads();
+  if (s_flag != 0u) {
+    if (tid == 0) {
+      a.bounds[a.k] = -1;
+      *a.flag = 0u;
+    }
+    return;
+  }
+  const int64_t len = (int64_t)a.k * a.n;
+  uint64_t carry = 0;
+  for (int64_t c0 = 0; c0 < len; c0 += kChunk) {
+    const int m = (int)(len - c0 < kChunk ? len - c0 : kChunk);
+    for (int e = tid; e < m; e += 1024) s_off[e] = a.counts[c0 + e];
+    __syncthreads();
+    const uint32_t tot = small_scan(s_off, m, s_wsum);
+    for (int e = tid; e < m; e += 1024) a.offs[c0 + e] = s_off[e] + (uint32_t)carry;
+    if (tid < a.k) {
+      const int64_t p = (int64_t)tid * a.n;  // first entry of list tid
+      if (p >= c0 && p < c0 + m) s_b[tid] = (int64_t)carry + s_off[p - c0];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (tid <= a.k) a.bounds[tid] = tid < a.k ? s_b[tid] : (int64_t)carry;
+}
+
+template <int NQ, bool FULL, bool DIST, bool DISP>
+__global__ __launch_bounds__(64 * kBatchRows) void k_batch_brute_fill(BatchSmallArgs a) {
+  __shared__ SysDesc s_d;
+  const int64_t total = a.bounds[a.k];
+  if (total < 0 || total > a.capacity) return;
+  const int tid = threadIdx.x;
+  const int b = (int)blockIdx.x;
+  const int s = uniform_i32(a.blk_sys[b]);
+  if (tid == 0) s_d = a.planned[s];
+  __syncthreads();
+  const int i = (b - a.blk_start[s]) * kBatchRows + uniform_i32(tid >> 6);
+  if (i >= s_d.n_atoms) return;
+  const int64_t u = s_d.atom_start + i;
+  PairArgs A{};
+  A.pos = a.pos;
+  A.global_index = a.prm.global_index;
+  A.edges = a.edges;
+  A.shifts = a.shifts;
+  A.dist = a.dist;
+  A.disp = a.disp;
+  Cuts<NQ> C = small_cuts<NQ>(a.rc2, a.k, a.disjoint);
+  int64_t e0[NQ];
+#pragma unroll
+  for (int g = 0; g < NQ; ++g) {
+    C.base[g] = g < a.k ? a.bounds[g] : 0;
+    C.len[g] = g < a.k ? a.bounds[g + 1] - a.bounds[g] : 0;
+    e0[g] = g < a.k ? (int64_t)a.offs[(int64_t)g * a.n + u] : 0;
+  }
+  Out<NQ, DIST, DISP> O;
+  O.init(A, C, e0);
+  unit_bruteforce<true, NQ, FULL, DIST, DISP>(A, C, &s_d, u, s_d.mode, O);
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan of u32 counts into int64 offsets, decoupled look-back.
 // bounds[q] = offset of element q*seg (q < nseg), bounds[nseg] = total.
 __global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
@@ -2235,6 +2362,50 @@ static void launch_brute_nq(bool fill, const SmallArgs& a, hipStream_t st) {
     else
       brute_fill_disp<NQ, false, false>(g, b, a, st);
   }
+}
+
+template <int NQ, bool FULL, bool DIST>
+static void batch_fill_disp(const dim3& g, const dim3& b, const BatchSmallArgs& a,
+                            hipStream_t st) {
+  if (a.disp)
+    hipLaunchKernelGGL((k_batch_brute_fill<NQ, FULL, DIST, true>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_batch_brute_fill<NQ, FULL, DIST, false>), g, b, 0, st, a);
+}
+template <int NQ>
+static void launch_batch_brute_nq(bool fill, const BatchSmallArgs& a, int blocks,
+                                  hipStream_t st) {
+  const dim3 g((unsigned)blocks), b(64 * kBatchRows);
+  if (!fill) {
+    if (a.full)
+      hipLaunchKernelGGL((k_batch_brute_count<NQ, true>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_batch_brute_count<NQ, false>), g, b, 0, st, a);
+    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, a);
+  } else if (a.full) {
+    if (a.dist)
+      batch_fill_disp<NQ, true, true>(g, b, a, st);
+    else
+      batch_fill_disp<NQ, true, false>(g, b, a, st);
+  } else {
+    if (a.dist)
+      batch_fill_disp<NQ, false, true>(g, b, a, st);
+    else
+      batch_fill_disp<NQ, false, false>(g, b, a, st);
+  }
+}
+
+hipError_t launch_batch_brute(bool fill, const BatchSmallArgs& a, int blocks, hipStream_t st) {
+  if (a.k < 1 || a.k > kGroup || a.n_sys < 1 || a.n < 1 || blocks < 1 || !a.sys ||
+      !a.blk_start || !a.blk_sys || !a.counts || !a.offs || !a.planned || !a.flag || !a.bounds)
+    return hipErrorInvalidValue;
+  if (a.k == 1)
+    launch_batch_brute_nq<1>(fill, a, blocks, st);
+  else if (a.k <= 4)
+    launch_batch_brute_nq<4>(fill, a, blocks, st);
+  else
+    launch_batch_brute_nq<8>(fill, a, blocks, st);
+  return hipGetLastError();
 }
 
 static bool small_args_ok(const SmallArgs& a) {

@@ -136,7 +136,7 @@ def _empty():
     return {"edge_index": np.zeros((2, 0), np.int64), "shift": np.zeros((0, 3), np.int32)}
 
 
-_PINNED_MIN = 1 << 20  # bytes; smaller lists are copied into ordinary numpy memory
+_PINNED_MIN = 1 << 16  # bytes; smaller lists are copied into ordinary numpy memory
 
 
 class _Pinned:
