@@ -120,6 +120,13 @@ int nl_get_small_path(void);
  * order of edges within a list). */
 void nl_set_bin_order(int morton);
 int nl_get_bin_order(void);
+/* Build stages of a batch whose systems all have at most 4096 atoms (linear
+ * bin order): 1 (the default) runs them as one launch, one block per system
+ * (plan, counting sort by bin in LDS, bin offsets, sorted positions, work
+ * items); 0 as the separate launches of the general path.  Never changes a
+ * result (the lists are byte-identical). */
+void nl_set_fused_build(int on);
+int nl_get_fused_build(void);
 void nl_set_fp32_filter(int on);
 int nl_get_fp32_filter(void);
 /* Work decomposition of the cell-list pair kernels: `group` home atoms of

@@ -553,6 +553,8 @@ void nl_set_small_path(int mode) {
 int nl_get_small_path(void) { return nl::config().small_path.load(); }
 
 void nl_set_bin_order(int morton) { nl::config().bin_order = morton ? 1 : 0; }
+void nl_set_fused_build(int on) { nl::config().fused_build = on ? 1 : 0; }
+int nl_get_fused_build(void) { return nl::config().fused_build.load(); }
 int nl_get_bin_order(void) { return nl::config().bin_order.load(); }
 void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }
 int nl_get_fp32_filter(void) { return nl::config().use_fp32.load(); }

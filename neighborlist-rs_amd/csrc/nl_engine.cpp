@@ -173,6 +173,7 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
       max_cap = std::max<int64_t>(max_cap, desc[s].bin_cap);
     }
     seg_sort = max_n <= kSegMax;
+    fused_build = seg_sort && config().fused_build.load() != 0;
     const int lb = bits_for(max_cap - 1);
     seg_passes = (lb + kSegBits - 1) / kSegBits;
   }
@@ -319,10 +320,52 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
 
 hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const {
   uint64_t* sstatus = (uint64_t*)b.zero;
+  uint32_t* tickets = b.zero + 2 * stiles + 2 * rstiles * passes;
+  if (fused_build && !b.bin_rank) {
+    // one launch builds every system (one block each): plan, sort, bin
+    // offsets, sorted positions, work list, zero counts of idle items
+    NL_TRY(launch_zero2(b.zero, z_words, nullptr, 0, st));
+    BuildArgs ba{};
+    ba.pos = b.pos;
+    ba.sys = b.sys;
+    ba.band = b.band;
+    ba.rc2 = b.rc2;
+    ba.prm = prm;
+    ba.item_base = b.items;
+    ba.n_items = n_items;
+    ba.group = tune.group;
+    ba.parts = tune.parts;
+    ba.keys = b.keysB;
+    ba.vals = b.valsB;
+    ba.cell_start = b.cell_start;
+    ba.spos64 = b.spos64;
+    ba.spos32 = b.spos32;
+    ba.sidx = b.sidx;
+    ba.list = b.list;
+    ba.n_real = tickets + 6;
+    ba.counts = b.counts;
+    NL_TRY(launch_build_systems(ba, S, st));
+  } else {
+    // zero block and counts (items without work keep zero counts)
+    NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st));
+    NL_TRY(enqueue_build_stages(b, st));
+  }
+  PairArgs A = pair_args(*this, b);
+  for (int q0 = 0; q0 < K; q0 += kGroup) {
+    A.q_lo = q0;
+    A.q_cnt = std::min(kGroup, K - q0);
+    NL_TRY(launch_pairs(false, A, st));
+  }
+  NL_TRY(launch_scan(b.counts, b.offsets, scan_len, n_items, K, b.bounds, sstatus, tickets + 4,
+                     st));
+  return hipSuccess;
+}
+
+// The build as separate launches (global radix sort, or Morton-ranked bins):
+// plan, bins, sort, bin offsets and sorted positions, work list.
+hipError_t Pipeline::enqueue_build_stages(const Buffers& b, hipStream_t st) const {
   uint64_t* rstatus = (uint64_t*)(b.zero + 2 * stiles);
   uint32_t* tickets = b.zero + 2 * stiles + 2 * rstiles * passes;
-  // zero block and counts (items without work keep zero counts)
-  NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st));
   NL_TRY(launch_plan(b.pos, b.sys, b.band, b.rc2, prm, st));
   if (b.bin_rank) NL_TRY(launch_morton(b.sys, S, bin_total, b.bin_rank, b.rank_bin, st));
   NL_TRY(launch_bin(b.pos, b.sys, b.start, S, n, b.bin_rank, b.keysA, b.valsA, st));
@@ -345,14 +388,6 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
   // work list of the items that have work
   NL_TRY(launch_items(kin, b.cell_start, b.sys, b.items, S, n_items, tune.group, tune.parts,
                       b.list, tickets + 6, st));
-  PairArgs A = pair_args(*this, b);
-  for (int q0 = 0; q0 < K; q0 += kGroup) {
-    A.q_lo = q0;
-    A.q_cnt = std::min(kGroup, K - q0);
-    NL_TRY(launch_pairs(false, A, st));
-  }
-  NL_TRY(launch_scan(b.counts, b.offsets, scan_len, n_items, K, b.bounds, sstatus, tickets + 4,
-                     st));
   return hipSuccess;
 }
 

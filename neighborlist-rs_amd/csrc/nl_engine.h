@@ -37,6 +37,8 @@ struct Config {
   // linear measured about 1% faster on the ragged batch (the rank lookups
   // cost more than the fewer cache misses save)
   std::atomic<int> bin_order{0};
+  // batches of systems of <= kSegMax atoms: the build stages in one launch
+  std::atomic<int> fused_build{1};
   Config();
 };
 Config& config();
@@ -123,6 +125,7 @@ class Pipeline {
   // Stages, all asynchronous on `st`.
   hipError_t enqueue_upload(const Buffers& b, const char* pinned, hipStream_t st) const;
   hipError_t enqueue_build_count(const Buffers& b, hipStream_t st) const;
+  hipError_t enqueue_build_stages(const Buffers& b, hipStream_t st) const;
   hipError_t enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
                           int64_t capacity, hipStream_t st, void* disp = nullptr) const;
   hipError_t enqueue_bounds(const Buffers& b, char* pinned, hipStream_t st) const;
@@ -135,6 +138,7 @@ class Pipeline {
   // every system fits one block: one segmented sort launch (seg_passes
   // digit passes in LDS) instead of the global radix passes
   bool seg_sort = false;
+  bool fused_build = false;  // seg_sort, as one launch (k_build_systems)
   int seg_passes = 0;
   Tuning tune;
   Params prm{};

@@ -96,6 +96,31 @@ struct BatchSmallArgs {
 // launch (fill = true)
 hipError_t launch_batch_brute(bool fill, const BatchSmallArgs& a, int blocks, hipStream_t st);
 
+// The build of a batch whose systems all have at most kSegMax atoms in one
+// launch (one block per system): plan, bins, sort, bin offsets, sorted
+// positions and wrap shifts, work list and zero counts of item slots without
+// work.  n_real must be zero before the launch.
+struct BuildArgs {
+  const double* pos;         // raw positions (n, 3), f64
+  SysDesc* sys;              // [n_sys]: descriptors in, plans out
+  float2* band;              // [n_sys][k] FP32 filter thresholds (written)
+  const double* rc2;         // [k], or [n_sys][k] with per-system cutoffs
+  Params prm;
+  const int64_t* item_base;  // [n_sys + 1]
+  int64_t n_items;
+  int32_t group, parts;
+  uint32_t* keys;            // sorted bin keys (bin_base + bin)
+  uint32_t* vals;            // sorted atoms (global index)
+  int32_t* cell_start;       // [total bins + 1]
+  Pos64* spos64;
+  float4* spos32;
+  int4* sidx;
+  int64_t* list;             // work list: group items from the front, rows from the back
+  uint32_t* n_real;          // [2] lengths of both ends
+  uint32_t* counts;          // [k][n_items]
+};
+hipError_t launch_build_systems(const BuildArgs& a, int n_sys, hipStream_t st);
+
 // one block (k_small)
 hipError_t launch_small(const SmallArgs& a, hipStream_t st);
 // count launch (fill = false; writes bounds), then fill launch (fill = true)

@@ -102,6 +102,25 @@ __device__ inline void atomic_store_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Exclusive scan over one value per thread, any block of up to 16 waves;
+// s_tmp holds 16 words.  *total (if given) gets the block's sum.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int nw = (int)(blockDim.x >> 6);
+  const uint32_t incl = (uint32_t)wave_incl_scan_dpp((int)v);
+  if (lane == 63) s_tmp[w] = incl;
+  __syncthreads();
+  uint32_t pre = 0, all = 0;
+  for (int i = 0; i < nw; ++i) {
+    const uint32_t t = s_tmp[i];
+    pre += i < w ? t : 0u;
+    all += t;
+  }
+  if (total) *total = all;
+  __syncthreads();
+  return pre + incl - v;
+}
+
 // Exclusive scan over 256 values held one per thread (block of 256).
 __device__ inline uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*4*/,
                                                uint32_t* total) {
@@ -685,6 +704,180 @@ __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict
   for (int i = tid + 1; i < n; i += 256) {
     const int k = (int)s_k[cur][i], prev = (int)s_k[cur][i - 1];
     for (int b = prev + 1; b <= k; ++b) cell_start[(int64_t)base + b] = (int32_t)(a0 + i);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The whole build of a batch whose systems all have at most kSegMax atoms, one
+// block per system, in one launch: the plan (plan_system), the bin of every
+// atom (as k_bin), the sort by bin, the bin offsets, the sorted wrapped
+// positions and wrap shifts (as k_finalize), the system's work items (as
+// k_items) and zero counts for its item slots without work.  It replaces
+// k_plan, k_bin, k_sort_systems, k_finalize and k_items, and the clearing of
+// the counts, with the same outputs: the sort is a counting sort in LDS (bin
+// histogram, exclusive scan, scatter) whose bins are then put back in atom
+// order, which is the order the stable sort gives.  Bins come from the grid
+// the plan chose (at most bin_cap <= 2 kSegMax + 64 of them).  On the
+// 256-system ragged batch (rc 14, bins of rc/3) it takes 28 us where the
+// five launches took 54 us.
+constexpr int kSegBins = 2 * kSegMax + 64;
+
+constexpr int kBuildThreads = 1024;
+
+__global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
+  __shared__ SysDesc s_d;
+  __shared__ uint32_t s_key[kSegMax];   // bin of local atom i
+  __shared__ uint32_t s_any[kSegMax];   // slots of a bin in scatter order
+  __shared__ uint32_t s_val[kSegMax];   // local atom of sorted slot p
+  __shared__ uint32_t s_bin[kSegBins];  // counts, then starts, then ends of the bins
+  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_base[2];
+  constexpr int T = kBuildThreads;
+  const int tid = threadIdx.x;
+  const int s = blockIdx.x;
+  const int k = a.prm.k;
+  const double* rc2s = a.prm.per_sys_rc ? a.rc2 + (int64_t)s * k : a.rc2;
+  if (tid == 0) s_d = a.sys[s];
+  __syncthreads();
+  plan_system(a.pos, &s_d, a.band + (int64_t)s * k, rc2s, a.prm);
+  __syncthreads();
+  if (tid == 0) a.sys[s] = s_d;  // the plan, for the pair kernels
+  const int64_t a0 = s_d.atom_start;
+  const int n = s_d.n_atoms;
+  const uint32_t bb = (uint32_t)s_d.bin_base;
+  const int cap = s_d.bin_cap;
+  const bool cells = s_d.mode == kCellList;
+  if (!cells) {
+    // brute force: every atom in bin 0 in input order (as the stable sort
+    // of equal keys leaves it)
+    for (int i = tid; i < n; i += T) {
+      a.keys[a0 + i] = bb;
+      a.vals[a0 + i] = (uint32_t)(a0 + i);
+    }
+    for (int b = tid; b <= cap; b += T) a.cell_start[(int64_t)bb + b] = (int32_t)(b == 0 ? a0 : a0 + n);
+  } else {
+    const int nb0 = s_d.nb[0], nb1 = s_d.nb[1], nb2 = s_d.nb[2];
+    const int nbins = nb0 * nb1 * nb2;  // <= bin_cap
+    for (int b = tid; b < nbins; b += T) s_bin[b] = 0u;
+    __syncthreads();
+    for (int i = tid; i < n; i += T) {
+      const double* q = a.pos + 3 * (a0 + i);
+      double f[3];
+      matvec3(s_d.hinv, q[0], q[1], q[2], f);
+      int c[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double u = f[d] - floor(f[d]);
+        const int v = (int)(u * (double)s_d.nb[d]);  // u >= 0
+        c[d] = v < s_d.nb[d] - 1 ? v : s_d.nb[d] - 1;
+      }
+      const uint32_t lin = (uint32_t)(c[0] + nb0 * (c[1] + nb1 * c[2]));
+      s_key[i] = lin;
+      atomicAdd(&s_bin[lin], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: thread t takes a contiguous run of bins
+    const int per = (nbins + T - 1) / T;
+    const int b0 = tid * per, b1 = min(nbins, b0 + per);
+    uint32_t sum = 0;
+    for (int b = b0; b < b1; ++b) sum += s_bin[b];
+    uint32_t run = block_excl_scan(sum, s_tmp, nullptr);
+    for (int b = b0; b < b1; ++b) {
+      const uint32_t c = s_bin[b];
+      s_bin[b] = run;
+      a.cell_start[(int64_t)bb + b] = (int32_t)(a0 + run);
+      run += c;
+    }
+    for (int b = nbins + tid; b <= cap; b += T) a.cell_start[(int64_t)bb + b] = (int32_t)(a0 + n);
+    __syncthreads();
+    // scatter (any order within a bin), then each atom to its place in atom
+    // order within its bin: its rank among the bin's atoms, one thread per
+    // slot (an insertion sort in one thread per bin cost 95 us per launch on
+    // the ragged batch, whose small periodic systems keep all their atoms in
+    // one or a few bins); afterwards s_bin[b] is the end of bin b
+    for (int i = tid; i < n; i += T) s_any[atomicAdd(&s_bin[s_key[i]], 1u)] = (uint32_t)i;
+    __syncthreads();
+    for (int p = tid; p < n; p += T) {
+      const uint32_t v = s_any[p], key = s_key[v];
+      const int lo = key > 0 ? (int)s_bin[key - 1] : 0, hi = (int)s_bin[key];
+      int r = 0;
+      for (int x = lo; x < hi; ++x) r += s_any[x] < v ? 1 : 0;
+      s_val[lo + r] = v;
+    }
+    __syncthreads();
+    // sorted keys and atoms; wrapped positions and wrap shifts of each
+    // slot, as CellList::build computes them (search.rs:189-204)
+    for (int p = tid; p < n; p += T) {
+      const int i = (int)s_val[p];
+      const int64_t at = a0 + i, slot = a0 + p;
+      a.keys[slot] = bb + s_key[i];
+      a.vals[slot] = (uint32_t)at;
+      const double* q = a.pos + 3 * at;
+      double f[3];
+      matvec3(s_d.hinv, q[0], q[1], q[2], f);
+      int32_t si[3];
+      for (int d = 0; d < 3; ++d) si[d] = as_i32(-floor(f[d]));
+      double sv[3];
+      matvec3(s_d.h, (double)(-si[0]), (double)(-si[1]), (double)(-si[2]), sv);
+      const double wx = q[0] - sv[0], wy = q[1] - sv[1], wz = q[2] - sv[2];
+      a.spos64[slot] = Pos64{wx, wy, wz, __hiloint2double(0, i)};  // w: index bits
+      a.spos32[slot] = make_float4((float)wx, (float)wy, (float)wz, __int_as_float(i));
+      a.sidx[slot] = make_int4(si[0], si[1], si[2], i);
+    }
+  }
+  // work items of the system (items [item0, item0 + n (1 + parts))): its
+  // rows on the brute-force path; on the cell-list path the stencil slices
+  // of every slot that starts a home group (up to `group` consecutive slots
+  // of one bin, aligned to the bin start).  Group items go to the front of
+  // the list, rows to the back; the other item slots get zero counts.
+  const int parts = a.parts, group = a.group;
+  const int64_t item0 = a.item_base[s];
+  auto starts_group = [&](int p) {
+    const int lo = s_key[s_val[p]] > 0 ? (int)s_bin[s_key[s_val[p]] - 1] : 0;
+    return (p - lo) % group == 0;
+  };
+  // thread t takes a contiguous run of slots, so that the list holds the
+  // system's items in slot order (neighbouring home groups read the same
+  // stencil bins and run close together); systems follow each other in the
+  // order their blocks finish
+  __syncthreads();
+  const int run_len = (n + T - 1) / T;
+  const int p_lo = min(n, tid * run_len), p_hi = min(n, p_lo + run_len);
+  uint32_t my_g = 0, my_r = 0;
+  if (cells) {
+    for (int p = p_lo; p < p_hi; ++p) my_g += starts_group(p) ? (uint32_t)parts : 0u;
+  } else {
+    my_r = (uint32_t)(p_hi - p_lo);
+  }
+  uint32_t tot_g = 0, tot_r = 0;
+  const uint32_t off_g = block_excl_scan(my_g, s_tmp, &tot_g);
+  const uint32_t off_r = block_excl_scan(my_r, s_tmp, &tot_r);
+  if (tid == 0) {
+    s_base[0] = tot_g ? atomicAdd(a.n_real, tot_g) : 0u;
+    s_base[1] = tot_r ? atomicAdd(a.n_real + 1, tot_r) : 0u;
+  }
+  __syncthreads();
+  uint32_t wg = s_base[0] + off_g, wr = s_base[1] + off_r;
+  const int64_t n_items = a.n_items;
+  for (int p = p_lo; p < p_hi; ++p) {
+    const int64_t row = item0 + p;
+    const int64_t cell0 = item0 + n + (int64_t)p * parts;
+    bool grp = false;
+    if (cells) {
+      grp = starts_group(p);
+    } else {
+      a.list[n_items - 1 - wr++] = row;
+    }
+    if (grp) {
+      for (int t = 0; t < parts; ++t) a.list[wg++] = cell0 + t;
+    }
+    // zero counts for the slots without work
+    for (int q = 0; q < k; ++q) {
+      uint32_t* cq = a.counts + (int64_t)q * n_items;
+      if (cells) cq[row] = 0u;
+      if (!grp)
+        for (int t = 0; t < parts; ++t) cq[cell0 + t] = 0u;
+    }
   }
 }
 
@@ -2481,6 +2674,13 @@ hipError_t launch_sort_systems(const uint32_t* kin, const uint32_t* vin, uint32_
   if (n_sys <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sort_systems, dim3((unsigned)n_sys), dim3(256), 0, st, kin, vin, kout, vout,
                      sys, sys_start, passes, cell_start);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_systems(const BuildArgs& a, int n_sys, hipStream_t st) {
+  if (n_sys <= 0) return hipSuccess;
+  if (a.prm.k < 1 || a.group < 1 || a.parts < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_build_systems, dim3((unsigned)n_sys), dim3(kBuildThreads), 0, st, a);
   return hipGetLastError();
 }
 

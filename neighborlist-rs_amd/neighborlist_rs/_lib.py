@@ -97,6 +97,8 @@ _SIGS = {
     "nl_set_small_path": (None, [C.c_int]),
     "nl_get_small_path": (C.c_int, []),
     "nl_set_bin_order": (None, [C.c_int]),
+    "nl_set_fused_build": (None, [C.c_int]),
+    "nl_get_fused_build": (C.c_int, []),
     "nl_get_bin_order": (C.c_int, []),
     "nl_set_fp32_filter": (None, [C.c_int]),
     "nl_get_fp32_filter": (C.c_int, []),

@@ -372,3 +372,98 @@ def test_segmented_sort_boundary_vs_oracle(nr, oracle, sizes):
     got = nr.build_neighborlists_batch_multi(p, batch, rows, cut)
     for k in range(len(cut)):
         assert_same_edges(as_tuple(got[k]), want[k], f"{sizes} list {k}")
+
+
+def _search_ex_device(lib, L, dpos, n, sizes, H9, has, pb, cut, multi, disjoint, opts):
+    """nl_search_ex on device positions: every array of every list as numpy."""
+    r = C.c_void_p()
+    assert lib.nl_search_ex(C.c_void_p(dpos), n, sizes.ctypes.data_as(L.i64p), sizes.size,
+                            H9.ctypes.data_as(L.dp), has.ctypes.data_as(L.u8p),
+                            pb.ctypes.data_as(L.u8p), cut.ctypes.data_as(L.dp), cut.size,
+                            int(disjoint), int(multi), opts | 1, None, C.byref(r)) == 0, \
+        L.last_error()
+    out = []
+    for q in range(cut.size):
+        m = int(lib.nl_result_num_edges(r, q))
+        ei = np.zeros((2, m), np.int64)
+        sh = np.zeros((m, 3), np.int32)
+        if m:
+            assert lib.nl_result_copy(r, q, ei[0].ctypes.data_as(L.i64p),
+                                      ei[1].ctypes.data_as(L.i64p),
+                                      sh.ctypes.data_as(L.i32p)) == 0
+        d = np.zeros(m)
+        if m and opts & 8:
+            assert lib.nl_result_copy_dist(r, q, d.ctypes.data_as(L.dp)) == 0
+        rng = np.zeros(2 * sizes.size * cut.size, np.int64)
+        modes = np.zeros(sizes.size, np.int32)
+        if opts & 4:
+            assert lib.nl_result_system_ranges(r, rng.ctypes.data_as(L.i64p),
+                                               modes.ctypes.data_as(L.i32p)) == 0
+        out.append((ei, sh, d, rng, modes))
+    lib.nl_result_free(r)
+    return out
+
+
+@pytest.mark.parametrize("tuning", [(0, 8, 2), (3, 16, 4), (2, 3, 1), (1, 64, 3)])
+def test_fused_build_identical_to_stages(nr, oracle, tuning):
+    """Batches whose systems all have at most 4096 atoms build in one launch
+    (k_build_systems: plan, counting sort by bin, bin offsets, sorted
+    positions, work list); the separate launches of the general path
+    (nl_set_fused_build(0)) must give the same arrays, order included:
+    the 256-system ragged batch with three lists, a batch at the 4096-atom
+    boundary with isolated, orthorhombic and triclinic systems and single
+    atoms (full lists, distances, ranges), per-system cutoffs are covered by
+    the serving tests.  One case is also checked against the oracle."""
+    from neighborlist_rs import _lib as L, workloads as W
+    import bench
+    lib = L.lib
+    cd, g, parts = tuning
+    olds = (lib.nl_get_autotune(), lib.nl_get_cell_div(), lib.nl_get_fused_build())
+    og, op = C.c_int(), C.c_int()
+    lib.nl_get_work_split(C.byref(og), C.byref(op))
+    lib.nl_set_autotune(0)
+    lib.nl_set_cell_div(cd)
+    lib.nl_set_work_split(g, parts)
+    try:
+        p4, s4, c4, cut4 = W.c4_ragged_batch()
+        rng = np.random.default_rng(4096)
+        sizes_b = [4096, 1, 4000, 37, 1, 900]
+        ps, cb = [], []
+        for s, m in enumerate(sizes_b):
+            p, H = W.random_system(rng, m, s % 3)
+            ps.append(p)
+            cb.append(H)
+        pb_ = np.ascontiguousarray(np.concatenate(ps))
+        cases = [("ragged batch", p4, np.asarray(s4, np.int64), c4, np.array(cut4), True, False,
+                  0),
+                 ("boundary batch, full lists, distances, ranges", pb_,
+                  np.array(sizes_b, np.int64), cb, np.array([3.5, 6.0]), True, True,
+                  16 | 8 | 4)]
+        for name, pos, sizes, cells, cut, multi, disjoint, opts in cases:
+            _print_case("fused build", tuning, name)
+            H9, has, pbc = bench.cell_arrays(cells)
+            d = bench.DevArray(lib, pos)
+            try:
+                lib.nl_set_fused_build(1)
+                a = _search_ex_device(lib, L, d.ptr, pos.shape[0], sizes, H9, has, pbc, cut,
+                                      multi, disjoint, opts)
+                lib.nl_set_fused_build(0)
+                b = _search_ex_device(lib, L, d.ptr, pos.shape[0], sizes, H9, has, pbc, cut,
+                                      multi, disjoint, opts)
+            finally:
+                d.free()
+            for q in range(cut.size):
+                for x, y in zip(a[q], b[q]):
+                    np.testing.assert_array_equal(x, y, err_msg=f"{name} list {q}")
+            if name == "ragged batch" and tuning[0] == 0:
+                batch, _ = W.batch_arrays(sizes, cells)
+                ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+                want = oracle.search_batch_multi(pos, batch, ocells, list(cut), False)
+                for q in range(cut.size):
+                    ei, sh = a[q][0], a[q][1]
+                    assert_same_edges((ei[0], ei[1], sh), want[q], f"fused list {q}")
+    finally:
+        lib.nl_set_autotune(olds[0])
+        lib.nl_set_cell_div(olds[1])
+        lib.nl_set_fused_build(olds[2])
+        lib.nl_set_work_split(og.value, op.value)
