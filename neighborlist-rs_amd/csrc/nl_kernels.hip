@@ -58,17 +58,21 @@ __device__ inline int find_sys(const int64_t* sys_start, int n_sys, int64_t a) {
   return lo;
 }
 
-// Inclusive scan of one 32-bit value per lane over the wave, by DPP lane
-// moves (row shifts 1, 2, 4, 8 inside each row of 16 lanes, then the row
-// broadcasts of lanes 15 and 31): no shuffle addresses in registers.
-__device__ __forceinline__ int wave_incl_max_dpp(int v) {
-  // as wave_incl_scan_dpp, with max; lanes without a source keep -1
-  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));  // row_shr:1
-  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));  // row_shr:2
-  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));  // row_shr:4
-  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));  // row_shr:8
-  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
-  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+// Inclusive scans over the wave by DPP lane moves (row shifts 1, 2, 4, 8
+// inside each row of 16 lanes, then the row broadcasts of lanes 15 and 31):
+// no shuffle addresses in registers.
+//
+// Running maximum of unsigned values with 0 as the empty value: every DPP
+// move fills missing sources with 0 (bound_ctrl, or the row masks of the
+// broadcasts keep the old 0), the identity of the unsigned maximum, so each
+// move can fold into its v_max_u32.
+__device__ __forceinline__ uint32_t wave_incl_umax_dpp(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));  // row_shr:1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));  // row_shr:2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));  // row_shr:4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));  // row_shr:8
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
   return v;
 }
 
@@ -1400,24 +1404,28 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
           valid = idx < total;
           const uint64_t before = __ballot(len > 0 && excl <= c0);
           const int s0 = before ? 63 - (int)__builtin_clzll(before) : 0;
-          L.mark[lane] = -1;
+          // marks hold segment + 1 (0: no segment starts at that lane)
+          L.mark[lane] = 0;
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          if (len > 0 && excl > c0 && excl <= c0 + 63) L.mark[excl - c0] = lane;
+          if (len > 0 && excl > c0 && excl <= c0 + 63) L.mark[excl - c0] = lane + 1;
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          sg = max(s0, wave_incl_max_dpp(L.mark[lane]));
+          sg = max(s0, (int)wave_incl_umax_dpp((uint32_t)L.mark[lane]) - 1);
           j = __shfl(base, sg, 64) + idx;
         };
         auto fetch = [&]() {
           if (valid) {
+            // address = base + j * size as one 32 x 32 -> 64-bit multiply-add
+            // (j >= 0): no sign extension and 64-bit shift per chunk
+            const uint64_t uj = (uint32_t)j;
             if (F32)
-              n32 = A.spos32[j];
+              n32 = *reinterpret_cast<const float4*>(uj * 16u + (uint64_t)A.spos32);
             else
-              n64 = A.spos64[j];
-            if (FILL) nij = A.sidx[j];
+              n64 = *reinterpret_cast<const Pos64*>(uj * 32u + (uint64_t)A.spos64);
+            if (FILL) nij = *reinterpret_cast<const int4*>(uj * 16u + (uint64_t)A.sidx);
           }
         };
         locate(0);
