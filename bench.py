@@ -228,7 +228,9 @@ def serve_trace(L, n_req, slots, load):
     model) and, separately, copied into numpy arrays.  Latency: open loop, Poisson arrivals at `load`
     times the measured capacity; latency = submit -> lists complete on the
     device.  Plain: the same requests one call each through the plain path
-    (host input, lists left on the device)."""
+    (host input, lists left on the device).  Requests are submitted with
+    copy=False: the server reads the trace's arrays when it stages a batch
+    instead of copying each one at submit."""
     import threading
 
     import neighborlist_rs as nr
@@ -265,7 +267,8 @@ def serve_trace(L, n_req, slots, load):
 
     def closed(host):
         t0 = time.perf_counter()
-        ids = [srv.submit(c, p, rc) for c, (_, p, _, rc) in zip(cells, trace)]
+        # the trace's arrays are not modified: the server may read them in place
+        ids = [srv.submit(c, p, rc, copy=False) for c, (_, p, _, rc) in zip(cells, trace)]
         tot = 0
         for rid in ids:
             if host:
@@ -299,7 +302,7 @@ def serve_trace(L, n_req, slots, load):
             dt = arrivals[k] - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            ids[k] = srv.submit(c, p, rc)
+            ids[k] = srv.submit(c, p, rc, copy=False)
             ready.release()
 
     st0 = srv.stats()

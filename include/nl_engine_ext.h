@@ -210,6 +210,15 @@ nl_server* nl_server_create(int slots, int64_t max_atoms, int64_t max_systems,
                             int64_t edge_budget);
 int64_t nl_server_submit(nl_server* s, const double* positions, size_t n, const double* cell,
                          const uint8_t* pbc, double cutoff);
+/* nl_server_submit with flags.  NL_SUBMIT_BORROW: the server does not copy
+ * the positions at submit; it reads them from the caller's buffer when the
+ * request's batch is staged, so the buffer must stay valid and unchanged
+ * until nl_server_wait reports the request done or failed.  Saves a host
+ * copy of every request (submission is otherwise bound by that copy: about
+ * 10 GB/s on one host thread). */
+#define NL_SUBMIT_BORROW 1u
+int64_t nl_server_submit_ex(nl_server* s, const double* positions, size_t n, const double* cell,
+                            const uint8_t* pbc, double cutoff, unsigned flags);
 int nl_server_wait(nl_server* s, int64_t id, double timeout_s);
 int nl_server_result_info(nl_server* s, int64_t id, int64_t* n_edges, double* latency_s,
                           int* batch_size);

@@ -1014,11 +1014,20 @@ nl_server* nl_server_create_ex(int slots, int64_t max_atoms, int64_t max_systems
 
 int64_t nl_server_submit(nl_server* s, const double* positions, size_t n, const double* cell,
                          const uint8_t* pbc, double cutoff) {
+  return nl_server_submit_ex(s, positions, n, cell, pbc, cutoff, 0u);
+}
+
+int64_t nl_server_submit_ex(nl_server* s, const double* positions, size_t n, const double* cell,
+                            const uint8_t* pbc, double cutoff, unsigned flags) {
   if (!s || !positions) {
     nl::set_error("server or positions pointer is NULL");
     return -1;
   }
-  return s->s->submit(positions, (int64_t)n, cell, pbc, cutoff);
+  if (flags & ~NL_SUBMIT_BORROW) {
+    nl::set_error("submit flags: NL_SUBMIT_BORROW only");
+    return -1;
+  }
+  return s->s->submit(positions, (int64_t)n, cell, pbc, cutoff, (flags & NL_SUBMIT_BORROW) != 0);
 }
 
 int nl_server_wait(nl_server* s, int64_t id, double timeout_s) {

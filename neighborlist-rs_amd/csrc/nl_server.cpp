@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "nl_engine.h"
@@ -40,20 +41,16 @@ static double now_s() {
 
 // Expected half-list size: half of N times the atoms within rc of an atom
 // (number density times the sphere volume; images included for periodic
-// cells, at most N-1 partners for an isolated cluster).
-int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc) {
+// cells, at most N-1 partners for an isolated cluster).  An isolated
+// cluster's volume is its bounding box widened by rc (mn / mx: the box).
+static int64_t estimate_from_box(int64_t n, const double* h, const double* mn, const double* mx,
+                                 double rc) {
   double vol;
   if (h) {
     const double det = (h[0] * (h[4] * h[8] - h[7] * h[5]) - h[1] * (h[3] * h[8] - h[6] * h[5]) +
                         h[2] * (h[3] * h[7] - h[6] * h[4]));
     vol = std::fabs(det);
   } else {
-    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int64_t i = 0; i < n; ++i)
-      for (int k = 0; k < 3; ++k) {
-        mn[k] = std::min(mn[k], pos[3 * i + k]);
-        mx[k] = std::max(mx[k], pos[3 * i + k]);
-      }
     vol = 1.0;
     for (int k = 0; k < 3; ++k) vol *= (mx[k] - mn[k]) + rc;
   }
@@ -62,6 +59,37 @@ int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc)
   if (!h) per_atom = std::min(per_atom, (double)(n - 1));
   return (int64_t)(0.5 * (double)n * per_atom) + 16;
 }
+
+int64_t estimate_edges(const double* pos, int64_t n, const double* h, double rc) {
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  if (!h)
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        mn[k] = std::min(mn[k], pos[3 * i + k]);
+        mx[k] = std::max(mx[k], pos[3 * i + k]);
+      }
+  return estimate_from_box(n, h, mn, mx, rc);
+}
+
+// A request's positions, copied at submit into a page-locked block of the
+// host pool (nl_host_alloc): reused blocks take no page faults, and the slot
+// thread's copy into the batch's staging buffer runs at memory speed.
+// Ordinary memory when the pool has no block.
+struct PosBuf {
+  double* p = nullptr;
+  int64_t n = 0;
+  bool pinned = false;
+  void reset() {
+    if (p) {
+      if (pinned)
+        nl_host_free(p);
+      else
+        std::free(p);
+    }
+    p = nullptr;
+  }
+  ~PosBuf() { reset(); }
+};
 
 // Device output of one batch: int64 edges [2][cap], int32 shifts [cap][3],
 // then (when the server emits them) FP64 distances [cap] and displacements
@@ -83,7 +111,8 @@ struct OutBlock {
 
 struct SrvReq {
   int64_t id = 0;
-  std::vector<double> pos;
+  PosBuf pos;  // freed once the batch holding the request has run
+  const double* ext = nullptr;  // the caller's positions (borrowed), else pos.p
   int has_cell = 0;
   double h[9] = {0};
   uint8_t pbc[3] = {1, 1, 1};
@@ -202,7 +231,7 @@ Server::~Server() {
 }
 
 int64_t Server::submit(const double* pos, int64_t n, const double* h, const uint8_t* pbc,
-                       double cutoff) {
+                       double cutoff, bool borrow) {
   if (!(std::isfinite(cutoff) && cutoff > 0.0)) {
     set_error("Cutoff must be positive and finite");
     return -1;
@@ -219,14 +248,44 @@ int64_t Server::submit(const double* pos, int64_t n, const double* h, const uint
     }
   }
   SrvReq* r = new SrvReq();
-  r->pos.assign(pos, pos + 3 * n);
+  r->pos.n = n;
+  double* dst = nullptr;
+  if (borrow) {
+    r->ext = pos;
+  } else {
+    const size_t bytes = sizeof(double) * 3 * (size_t)n;
+    r->pos.p = (double*)nl_host_alloc(bytes);
+    r->pos.pinned = r->pos.p != nullptr;
+    if (!r->pos.p) r->pos.p = (double*)std::malloc(bytes);
+    if (!r->pos.p) {
+      delete r;
+      set_error("out of host memory for the request's positions");
+      return -1;
+    }
+    r->ext = r->pos.p;
+    dst = r->pos.p;
+  }
+  // copy (unless borrowed), with the bounding box of an isolated cluster in
+  // the same pass
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  if (h) {
+    if (dst) std::memcpy(dst, pos, sizeof(double) * 3 * (size_t)n);
+  } else {
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const double v = pos[3 * i + k];
+        if (dst) dst[3 * i + k] = v;
+        mn[k] = std::min(mn[k], v);
+        mx[k] = std::max(mx[k], v);
+      }
+  }
   if (h) {
     r->has_cell = 1;
     std::memcpy(r->h, h, sizeof r->h);
     for (int k = 0; k < 3; ++k) r->pbc[k] = pbc ? (pbc[k] != 0) : 1;
   }
   r->cutoff = cutoff;
-  r->est = estimate_edges(pos, n, h, cutoff) * (full_ ? 2 : 1);
+  r->est = estimate_from_box(n, h, mn, mx, cutoff) * (full_ ? 2 : 1);
   std::lock_guard<std::mutex> g(mu_);
   r->id = next_id_++;
   r->t_submit = now_s();
@@ -363,6 +422,8 @@ void Server::fail_slot(Slot& s, const std::string& msg) {
     r->state = 3;
     r->err = msg;
     r->t_done = now_s();
+    r->pos.reset();
+    r->ext = nullptr;
   }
   if (s.blk) pool_.push_back(s.blk);
   s.blk = nullptr;
@@ -420,7 +481,7 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   req.starts.push_back(0);
   double rcmax = 0;
   for (SrvReq* r : batch) {
-    n += (int64_t)(r->pos.size() / 3);
+    n += r->pos.n;
     req.starts.push_back(n);
     SysIn si;
     si.has_cell = r->has_cell;
@@ -468,8 +529,8 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   }
   size_t off = 0;
   for (SrvReq* r : batch) {
-    std::memcpy(s.hpos + off, r->pos.data(), 8 * r->pos.size());
-    off += r->pos.size();
+    std::memcpy(s.hpos + off, r->ext, 24 * (size_t)r->pos.n);
+    off += 3 * (size_t)r->pos.n;
   }
   s.P.stage_host(s.pin);
   int64_t est = 0;
@@ -537,6 +598,8 @@ void Server::deliver(Slot& s, int64_t total) {
   s.blk->refs = (int)s.reqs.size();
   for (size_t k = 0; k < s.reqs.size(); ++k) {
     SrvReq* r = s.reqs[k];
+    r->pos.reset();  // the batch's upload completed before its counts did
+    r->ext = nullptr;
     r->blk = s.blk;
     r->first = s.hranges[2 * k];
     r->len = s.hranges[2 * k + 1] - r->first;
@@ -572,7 +635,7 @@ void Server::slot_loop(Slot& s) {
       int64_t atoms = 0;
       while (!queue_.empty()) {
         SrvReq* r = queue_.front();
-        const int64_t na = (int64_t)(r->pos.size() / 3);
+        const int64_t na = r->pos.n;
         const bool first = batch.empty();
         if (!first && (atoms + na > max_atoms_ || (int64_t)batch.size() >= max_systems_))
           break;

@@ -25,8 +25,10 @@ class Server {
          bool full = false, bool dist = false, bool disp = false);
   ~Server();
   // returns the request id (> 0) or -1 with the thread's error message set
+  // borrow: read the caller's positions until the request completes instead
+  // of copying them at submit
   int64_t submit(const double* pos, int64_t n, const double* h, const uint8_t* pbc,
-                 double cutoff);
+                 double cutoff, bool borrow = false);
   int wait(int64_t id, double timeout_s);  // 0 done, 1 timed out, -1 failed
   int info(int64_t id, int64_t* n_edges, double* latency_s, int* batch_size);
   // device addresses of a completed request's lists (valid until release/take)

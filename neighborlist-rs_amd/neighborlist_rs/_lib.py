@@ -125,6 +125,8 @@ _SIGS = {
     "nl_plan_system_ranges": (C.c_int, [C.c_void_p, i64p, i32p]),
     "nl_server_create": (C.c_void_p, [C.c_int, C.c_int64, C.c_int64, C.c_int64]),
     "nl_server_submit": (C.c_int64, [C.c_void_p, dp, C.c_size_t, dp, u8p, C.c_double]),
+    "nl_server_submit_ex": (C.c_int64, [C.c_void_p, dp, C.c_size_t, dp, u8p, C.c_double,
+                                        C.c_uint]),
     "nl_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
     "nl_server_result_info": (C.c_int, [C.c_void_p, C.c_int64, i64p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int)]),

@@ -35,6 +35,7 @@ class Server:
     def __init__(self, slots=3, max_atoms=1 << 20, max_systems=1024, edge_budget=1 << 28, *,
                  full=False, distances=False, displacements=False):
         self._dist, self._disp = bool(distances), bool(displacements)
+        self._borrowed = {}  # request id -> positions the server reads until it is done
         opts = (16 if full else 0) | (8 if distances else 0) | (32 if displacements else 0)
         self._p = lib.nl_server_create_ex(int(slots), int(max_atoms), int(max_systems),
                                           int(edge_budget), opts)
@@ -45,6 +46,7 @@ class Server:
         if self._p:
             lib.nl_server_free(self._p)
             self._p = None
+        self._borrowed.clear()
 
     def __del__(self):
         self.close()
@@ -55,7 +57,11 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
-    def submit(self, cell, positions, cutoff):
+    def submit(self, cell, positions, cutoff, *, copy=True):
+        """Queues one request; returns its id.  copy=False lets the server
+        read `positions` when the request's batch is staged instead of copying
+        them now (the server keeps a reference until the request is done):
+        the array must not be modified in place before then."""
         from . import _cell_args, _positions
 
         cutoff = float(cutoff)
@@ -63,16 +69,19 @@ class Server:
             raise ValueError("Cutoff must be positive and finite")
         pos = _positions(positions)
         h, pb = _cell_args(cell)
-        rid = int(lib.nl_server_submit(self._p, pos.ctypes.data_as(_lib.dp), pos.shape[0], h, pb,
-                                       cutoff))
+        rid = int(lib.nl_server_submit_ex(self._p, pos.ctypes.data_as(_lib.dp), pos.shape[0], h,
+                                          pb, cutoff, 0 if copy else 1))
         if rid < 0:
             raise ValueError(_lib.last_error())
+        if not copy:
+            self._borrowed[rid] = pos
         return rid
 
     def wait(self, rid, timeout=None):
         rc = lib.nl_server_wait(self._p, int(rid), -1.0 if timeout is None else float(timeout))
         if rc == 1:
             return False
+        self._borrowed.pop(int(rid), None)  # done or failed: no longer read
         if rc != 0:
             raise RuntimeError(_lib.last_error())
         return True

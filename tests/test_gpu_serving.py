@@ -28,9 +28,9 @@ def _trace(seed=11, n=40, nmax=3000):
     return c5_serving_trace(seed=seed, n_requests=n, nmin=8, nmax=nmax)
 
 
-def _serve(nr, reqs, **kw):
+def _serve(nr, reqs, copy=True, **kw):
     with nr.Server(**kw) as srv:
-        ids = [srv.submit(_cell(nr, H), p, rc) for (_, p, H, rc) in reqs]
+        ids = [srv.submit(_cell(nr, H), p, rc, copy=copy) for (_, p, H, rc) in reqs]
         info = []
         out = []
         for rid in ids:
@@ -271,3 +271,23 @@ def test_serving_result_arrays(nr, oracle):
             te = c(t["edge_index"])
             o2 = np.lexsort((ts[:, 2], ts[:, 1], ts[:, 0], te[1], te[0]))
             np.testing.assert_array_equal(d[o1], c(t["distance"])[o2], err_msg=f"request {k}")
+
+
+def test_serving_borrowed_positions(nr, oracle):
+    """submit(copy=False) (NL_SUBMIT_BORROW): the server reads the caller's
+    positions when it stages the batch instead of copying them at submit;
+    the lists are the same arrays as with copied positions, and the server
+    holds no reference once a request is done."""
+    reqs = _trace(seed=23, n=30)
+    _print_case("serving borrowed positions")
+    copied, _, _ = _serve(nr, reqs, slots=3, max_atoms=20000, max_systems=64)
+    with nr.Server(slots=3, max_atoms=20000, max_systems=64) as srv:
+        ids = [srv.submit(_cell(nr, H), p, rc, copy=False) for (_, p, H, rc) in reqs]
+        borrowed = [srv.result(rid, timeout=60.0) for rid in ids]
+        assert not srv._borrowed
+    for k, (a, b) in enumerate(zip(copied, borrowed)):
+        np.testing.assert_array_equal(a["edge_index"], b["edge_index"], err_msg=f"request {k}")
+        np.testing.assert_array_equal(a["shift"], b["shift"], err_msg=f"request {k}")
+    _, p, H, rc = reqs[0]
+    want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
+    assert_same_edges(as_tuple(borrowed[0]), want, "request 0")
