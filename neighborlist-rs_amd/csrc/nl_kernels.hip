@@ -1418,14 +1418,16 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
         };
         auto fetch = [&]() {
           if (valid) {
-            // address = base + j * size as one 32 x 32 -> 64-bit multiply-add
-            // (j >= 0): no sign extension and 64-bit shift per chunk
+            // j >= 0: an unsigned index needs no sign extension.  Pointer
+            // arithmetic, not an integer address: the loads stay global
+            // loads (a flat load also counts against the LDS wait of every
+            // step of the home-atom loop, which would expose its latency)
             const uint64_t uj = (uint32_t)j;
             if (F32)
-              n32 = *reinterpret_cast<const float4*>(uj * 16u + (uint64_t)A.spos32);
+              n32 = A.spos32[uj];
             else
-              n64 = *reinterpret_cast<const Pos64*>(uj * 32u + (uint64_t)A.spos64);
-            if (FILL) nij = *reinterpret_cast<const int4*>(uj * 16u + (uint64_t)A.sidx);
+              n64 = A.spos64[uj];
+            if (FILL) nij = A.sidx[uj];
           }
         };
         locate(0);
