@@ -120,11 +120,14 @@ int nl_get_small_path(void);
  * order of edges within a list). */
 void nl_set_bin_order(int morton);
 int nl_get_bin_order(void);
-/* Build stages of a batch whose systems all have at most 4096 atoms (linear
- * bin order): 1 (the default) runs them as one launch, one block per system
- * (plan, counting sort by bin in LDS, bin offsets, sorted positions, work
- * items); 0 as the separate launches of the general path.  Never changes a
- * result (the lists are byte-identical). */
+/* Build stages of a batch whose systems all have at most 12288 atoms (linear
+ * bin order): one launch, one block per system (plan, counting sort by bin in
+ * LDS, bin offsets, sorted positions, work items), or the separate launches
+ * of the general path.  1 (the default) takes the one launch when every
+ * system has at most 4096 atoms, or at most 12288 and the batch holds at
+ * least four times its largest system's atoms (one block per system: a lone
+ * large system builds faster over many blocks); 2 whenever the systems fit;
+ * 0 never.  Never changes a result (the lists are byte-identical). */
 void nl_set_fused_build(int on);
 int nl_get_fused_build(void);
 void nl_set_fp32_filter(int on);
@@ -231,6 +234,10 @@ int nl_server_release(nl_server* s, int64_t id);
  * (their fill pass ran a second time into larger buffers) */
 int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t* max_inflight,
                     int64_t* max_batch, int64_t* refills);
+/* device and page-locked allocations the server made after its creation
+ * (output blocks, slot buffers that grew); freeing a device buffer
+ * synchronises the device, so a warmed-up server should report no new ones */
+int nl_server_allocations(nl_server* s, int64_t* n);
 /* waits for every submitted request, stops the slot threads, frees the buffers */
 void nl_server_free(nl_server* s);
 /* A server whose requests all get the outputs in `options` (NL_FULL: full

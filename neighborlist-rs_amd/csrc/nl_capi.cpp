@@ -553,7 +553,7 @@ void nl_set_small_path(int mode) {
 int nl_get_small_path(void) { return nl::config().small_path.load(); }
 
 void nl_set_bin_order(int morton) { nl::config().bin_order = morton ? 1 : 0; }
-void nl_set_fused_build(int on) { nl::config().fused_build = on ? 1 : 0; }
+void nl_set_fused_build(int on) { nl::config().fused_build = on == 2 ? 2 : (on ? 1 : 0); }
 int nl_get_fused_build(void) { return nl::config().fused_build.load(); }
 int nl_get_bin_order(void) { return nl::config().bin_order.load(); }
 void nl_set_fp32_filter(int on) { nl::config().use_fp32.store(on != 0); }
@@ -1107,6 +1107,15 @@ int nl_server_stats(nl_server* s, int64_t* completed, int64_t* batches, int64_t*
     return -1;
   }
   s->s->stats(completed, batches, max_inflight, max_batch, refills);
+  return 0;
+}
+
+int nl_server_allocations(nl_server* s, int64_t* n) {
+  if (!s || !n) {
+    nl::set_error("server or output pointer is NULL");
+    return -1;
+  }
+  *n = s->s->allocations();
   return 0;
 }
 

@@ -23,6 +23,8 @@ constexpr int kScanChainTiles = 32;
 // segmented sort of small systems: atoms per system, digit width
 constexpr int kSegMax = 4096;
 constexpr int kSegBits = 7;
+// one-launch build (k_build_systems): atoms per system (16-bit local indices)
+constexpr int kFusedMax = 12288;
 // single systems of at most this many atoms on the brute-force path: one
 // block plans, counts, scans and fills them in one launch (k_small)
 constexpr int kSmallRows = 1024;

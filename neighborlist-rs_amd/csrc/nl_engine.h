@@ -37,7 +37,7 @@ struct Config {
   // linear measured about 1% faster on the ragged batch (the rank lookups
   // cost more than the fewer cache misses save)
   std::atomic<int> bin_order{0};
-  // batches of systems of <= kSegMax atoms: the build stages in one launch
+  // batches of systems of <= kFusedMax atoms: the build stages in one launch
   std::atomic<int> fused_build{1};
   Config();
 };
@@ -138,7 +138,9 @@ class Pipeline {
   // every system fits one block: one segmented sort launch (seg_passes
   // digit passes in LDS) instead of the global radix passes
   bool seg_sort = false;
-  bool fused_build = false;  // seg_sort, as one launch (k_build_systems)
+  // every system has at most kFusedMax atoms: the whole build in one launch
+  // (k_build_systems)
+  bool fused_build = false;
   int seg_passes = 0;
   Tuning tune;
   Params prm{};

@@ -96,7 +96,7 @@ struct BatchSmallArgs {
 // launch (fill = true)
 hipError_t launch_batch_brute(bool fill, const BatchSmallArgs& a, int blocks, hipStream_t st);
 
-// The build of a batch whose systems all have at most kSegMax atoms in one
+// The build of a batch whose systems all have at most kFusedMax atoms in one
 // launch (one block per system): plan, bins, sort, bin offsets, sorted
 // positions and wrap shifts, work list and zero counts of item slots without
 // work.  n_real must be zero before the launch.

@@ -712,28 +712,42 @@ __global__ __launch_bounds__(256) void k_sort_systems(const uint32_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// The whole build of a batch whose systems all have at most kSegMax atoms, one
-// block per system, in one launch: the plan (plan_system), the bin of every
-// atom (as k_bin), the sort by bin, the bin offsets, the sorted wrapped
+// The whole build of a batch whose systems all have at most kFusedMax atoms,
+// one block per system, in one launch: the plan (plan_system), the bin of
+// every atom (as k_bin), the sort by bin, the bin offsets, the sorted wrapped
 // positions and wrap shifts (as k_finalize), the system's work items (as
 // k_items) and zero counts for its item slots without work.  It replaces
-// k_plan, k_bin, k_sort_systems, k_finalize and k_items, and the clearing of
-// the counts, with the same outputs: the sort is a counting sort in LDS (bin
-// histogram, exclusive scan, scatter) whose bins are then put back in atom
-// order, which is the order the stable sort gives.  Bins come from the grid
-// the plan chose (at most bin_cap <= 2 kSegMax + 64 of them).  On the
-// 256-system ragged batch (rc 14, bins of rc/3) it takes 28 us where the
-// five launches took 54 us.
-constexpr int kSegBins = 2 * kSegMax + 64;
+// k_plan, k_bin, the sort (k_sort_systems or the radix passes), k_finalize
+// and k_items, and the clearing of the counts, with the same outputs: the
+// sort is a counting sort in LDS (bin histogram, exclusive scan, scatter)
+// whose bins are then put back in atom order, which is the order the stable
+// sort gives.  Bins come from the grid the plan chose (at most bin_cap <=
+// 2 kFusedMax + 64 of them).  Local atom indices and bins fit 16 bits, so the
+// per-atom arrays are u16 and the bin counts are u16 halves of u32 words
+// (a count never exceeds the system's atoms, so an increment never carries
+// into the other half): 123 KB of LDS at 12288 atoms.  On the 256-system
+// ragged batch (rc 14, bins of rc/3) it takes 28 us where the five launches
+// took 54 us.
+constexpr int kFusedBins = 2 * kFusedMax + 64;
 
 constexpr int kBuildThreads = 1024;
 
+// bin b's count / offset in the packed halves
+__device__ __forceinline__ uint32_t bin_get(const uint32_t* w, int b) {
+  return (w[b >> 1] >> ((b & 1) << 4)) & 0xFFFFu;
+}
+// increments bin b, returns its value before
+__device__ __forceinline__ uint32_t bin_inc(uint32_t* w, int b) {
+  const int sh = (b & 1) << 4;
+  return (atomicAdd(&w[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
+}
+
 __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
   __shared__ SysDesc s_d;
-  __shared__ uint32_t s_key[kSegMax];   // bin of local atom i
-  __shared__ uint32_t s_any[kSegMax];   // slots of a bin in scatter order
-  __shared__ uint32_t s_val[kSegMax];   // local atom of sorted slot p
-  __shared__ uint32_t s_bin[kSegBins];  // counts, then starts, then ends of the bins
+  __shared__ uint16_t s_key[kFusedMax];        // bin of local atom i
+  __shared__ uint16_t s_any[kFusedMax];        // slots of a bin in scatter order
+  __shared__ uint16_t s_val[kFusedMax];        // local atom of sorted slot p
+  __shared__ uint32_t s_bin[kFusedBins / 2];   // counts, then starts, then ends (u16 pairs)
   __shared__ uint32_t s_tmp[16];
   __shared__ uint32_t s_base[2];
   constexpr int T = kBuildThreads;
@@ -762,7 +776,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
   } else {
     const int nb0 = s_d.nb[0], nb1 = s_d.nb[1], nb2 = s_d.nb[2];
     const int nbins = nb0 * nb1 * nb2;  // <= bin_cap
-    for (int b = tid; b < nbins; b += T) s_bin[b] = 0u;
+    for (int w = tid; w < (nbins + 1) / 2; w += T) s_bin[w] = 0u;
     __syncthreads();
     for (int i = tid; i < n; i += T) {
       const double* q = a.pos + 3 * (a0 + i);
@@ -775,22 +789,26 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
         const int v = (int)(u * (double)s_d.nb[d]);  // u >= 0
         c[d] = v < s_d.nb[d] - 1 ? v : s_d.nb[d] - 1;
       }
-      const uint32_t lin = (uint32_t)(c[0] + nb0 * (c[1] + nb1 * c[2]));
-      s_key[i] = lin;
-      atomicAdd(&s_bin[lin], 1u);
+      const int lin = c[0] + nb0 * (c[1] + nb1 * c[2]);
+      s_key[i] = (uint16_t)lin;
+      (void)bin_inc(s_bin, lin);
     }
     __syncthreads();
     // exclusive scan of the counts: thread t takes a contiguous run of bins
-    const int per = (nbins + T - 1) / T;
-    const int b0 = tid * per, b1 = min(nbins, b0 + per);
+    // that starts at an even bin (whole words: no two threads share one)
+    const int per = (((nbins + T - 1) / T) + 1) & ~1;
+    const int b0 = min(nbins, tid * per), b1 = min(nbins, b0 + per);
     uint32_t sum = 0;
-    for (int b = b0; b < b1; ++b) sum += s_bin[b];
+    for (int b = b0; b < b1; ++b) sum += bin_get(s_bin, b);
     uint32_t run = block_excl_scan(sum, s_tmp, nullptr);
-    for (int b = b0; b < b1; ++b) {
-      const uint32_t c = s_bin[b];
-      s_bin[b] = run;
-      a.cell_start[(int64_t)bb + b] = (int32_t)(a0 + run);
-      run += c;
+    for (int b = b0; b < b1; b += 2) {
+      const uint32_t wd = s_bin[b >> 1];
+      const uint32_t c0 = wd & 0xFFFFu, c1 = wd >> 16;
+      const uint32_t st0 = run, st1 = run + c0;
+      a.cell_start[(int64_t)bb + b] = (int32_t)(a0 + st0);
+      if (b + 1 < b1) a.cell_start[(int64_t)bb + b + 1] = (int32_t)(a0 + st1);
+      s_bin[b >> 1] = st0 | (st1 << 16);
+      run = st1 + (b + 1 < b1 ? c1 : 0u);
     }
     for (int b = nbins + tid; b <= cap; b += T) a.cell_start[(int64_t)bb + b] = (int32_t)(a0 + n);
     __syncthreads();
@@ -798,15 +816,16 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
     // order within its bin: its rank among the bin's atoms, one thread per
     // slot (an insertion sort in one thread per bin cost 95 us per launch on
     // the ragged batch, whose small periodic systems keep all their atoms in
-    // one or a few bins); afterwards s_bin[b] is the end of bin b
-    for (int i = tid; i < n; i += T) s_any[atomicAdd(&s_bin[s_key[i]], 1u)] = (uint32_t)i;
+    // one or a few bins); afterwards bin b holds its end
+    for (int i = tid; i < n; i += T) s_any[bin_inc(s_bin, s_key[i])] = (uint16_t)i;
     __syncthreads();
     for (int p = tid; p < n; p += T) {
       const uint32_t v = s_any[p], key = s_key[v];
-      const int lo = key > 0 ? (int)s_bin[key - 1] : 0, hi = (int)s_bin[key];
+      const int lo = key > 0 ? (int)bin_get(s_bin, (int)key - 1) : 0;
+      const int hi = (int)bin_get(s_bin, (int)key);
       int r = 0;
       for (int x = lo; x < hi; ++x) r += s_any[x] < v ? 1 : 0;
-      s_val[lo + r] = v;
+      s_val[lo + r] = (uint16_t)v;
     }
     __syncthreads();
     // sorted keys and atoms; wrapped positions and wrap shifts of each
@@ -814,7 +833,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
     for (int p = tid; p < n; p += T) {
       const int i = (int)s_val[p];
       const int64_t at = a0 + i, slot = a0 + p;
-      a.keys[slot] = bb + s_key[i];
+      a.keys[slot] = bb + (uint32_t)s_key[i];
       a.vals[slot] = (uint32_t)at;
       const double* q = a.pos + 3 * at;
       double f[3];
@@ -837,7 +856,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
   const int parts = a.parts, group = a.group;
   const int64_t item0 = a.item_base[s];
   auto starts_group = [&](int p) {
-    const int lo = s_key[s_val[p]] > 0 ? (int)s_bin[s_key[s_val[p]] - 1] : 0;
+    const int key = (int)s_key[s_val[p]];
+    const int lo = key > 0 ? (int)bin_get(s_bin, key - 1) : 0;
     return (p - lo) % group == 0;
   };
   // thread t takes a contiguous run of slots, so that the list holds the

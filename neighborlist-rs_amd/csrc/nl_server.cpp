@@ -149,11 +149,16 @@ struct Server::Slot {
   std::thread th;
 };
 
-static bool grow_pinned(void** p, size_t* cap, size_t bytes) {
+// Slot buffers grow to `target` (at least `bytes`, the batch's need):
+// launch() asks for the need scaled towards the server's largest batch, so a
+// slot reaches its working size in a few allocations, not one per new size.
+static bool grow_pinned(void** p, size_t* cap, size_t bytes, size_t target,
+                        std::atomic<int64_t>* n) {
   if (*cap >= bytes) return true;
+  ++*n;
   if (*p) (void)hipHostFree(*p);
   *p = nullptr;
-  const size_t c = std::max(bytes, *cap * 2);
+  const size_t c = std::max({bytes, target, *cap * 2});
   if (hipHostMalloc(p, c, hipHostMallocDefault) != hipSuccess) {
     *cap = 0;
     return false;
@@ -162,11 +167,13 @@ static bool grow_pinned(void** p, size_t* cap, size_t bytes) {
   return true;
 }
 
-static bool grow_dev(void** p, size_t* cap, size_t bytes) {
+static bool grow_dev(void** p, size_t* cap, size_t bytes, size_t target,
+                     std::atomic<int64_t>* n) {
   if (*cap >= bytes) return true;
+  ++*n;
   if (*p) (void)hipFree(*p);
   *p = nullptr;
-  const size_t c = std::max(bytes, *cap + *cap / 2);
+  const size_t c = std::max({bytes, target, *cap + *cap / 2});
   if (hipMalloc(p, c) != hipSuccess) {
     *cap = 0;
     return false;
@@ -439,7 +446,17 @@ void Server::fail_slot(Slot& s, const std::string& msg) {
 // An output block for at least `cap` edges: the smallest pooled block that
 // fits, else a new allocation (when the pool already holds more than a few
 // blocks, its smallest one is freed first).
+// Output blocks come in capacity classes (2^20 edges, then steps of 1.5x):
+// batches of similar size share blocks, so a warmed-up server allocates and
+// frees none (hipFree synchronises the whole device).
+static int64_t block_class(int64_t cap) {
+  int64_t c = int64_t(1) << 20;
+  while (c < cap) c += c / 2;
+  return c;
+}
+
 OutBlock* Server::get_block(int64_t cap) {
+  cap = block_class(cap);
   OutBlock* victim = nullptr;
   {
     std::lock_guard<std::mutex> g(mu_);
@@ -451,7 +468,7 @@ OutBlock* Server::get_block(int64_t cap) {
       pool_.erase(pool_.begin() + bi);
       return b;
     }
-    if (pool_.size() > slots_.size() + 2) {
+    if (pool_.size() > 2 * slots_.size() + 2) {
       auto it = std::min_element(pool_.begin(), pool_.end(),
                                  [](OutBlock* a, OutBlock* b) { return a->cap < b->cap; });
       victim = *it;
@@ -462,6 +479,7 @@ OutBlock* Server::get_block(int64_t cap) {
     (void)hipFree(victim->dev);
     delete victim;
   }
+  ++allocs_;
   OutBlock* b = new OutBlock();
   b->cap = cap;
   b->dist = dist_;
@@ -513,17 +531,26 @@ bool Server::launch(Slot& s, std::vector<SrvReq*>& batch) {
   }
   std::vector<size_t> sz = s.P.buffer_sizes();
   sz[0] = sizeof(double) * 3 * (size_t)n;
-  for (int i = 0; i < kNumBuffers; ++i)
-    if (!grow_dev(&s.bufs[i], &s.caps[i], std::max<size_t>(sz[i], 16))) {
+  const size_t S = batch.size();
+  // every buffer grows with the batch's atoms or systems: scale a need
+  // towards the largest batch the server admits (at most 16x at a time)
+  const double scale = std::min(
+      16.0, std::max({1.0, (double)max_atoms_ / (double)std::max<int64_t>(n, 1),
+                      (double)max_systems_ / (double)std::max<size_t>(S, 1)}));
+  auto target = [scale](size_t need) { return (size_t)((double)need * scale); };
+  for (int i = 0; i < kNumBuffers; ++i) {
+    const size_t need = std::max<size_t>(sz[i], 16);
+    if (!grow_dev(&s.bufs[i], &s.caps[i], need, target(need), &allocs_)) {
       fail_slot(s, "out of device memory in the server");
       return false;
     }
+  }
   s.P.assign(s.B, s.bufs);
-  const size_t S = batch.size();
-  if (!grow_pinned((void**)&s.pin, &s.pin_cap, s.P.pinned_bytes()) ||
-      !grow_pinned((void**)&s.hpos, &s.hpos_cap, sizeof(double) * 3 * (size_t)n) ||
-      !grow_pinned((void**)&s.hranges, &s.hranges_cap, 16 * S) ||
-      !grow_dev((void**)&s.dranges, &s.dranges_cap, 16 * S)) {
+  const size_t pin_need = s.P.pinned_bytes(), hpos_need = sizeof(double) * 3 * (size_t)n;
+  if (!grow_pinned((void**)&s.pin, &s.pin_cap, pin_need, target(pin_need), &allocs_) ||
+      !grow_pinned((void**)&s.hpos, &s.hpos_cap, hpos_need, target(hpos_need), &allocs_) ||
+      !grow_pinned((void**)&s.hranges, &s.hranges_cap, 16 * S, target(16 * S), &allocs_) ||
+      !grow_dev((void**)&s.dranges, &s.dranges_cap, 16 * S, target(16 * S), &allocs_)) {
     fail_slot(s, "out of memory in the server");
     return false;
   }

@@ -1,5 +1,6 @@
 // nl_server.h -- continuous-batching request server (see nl_server.cpp).
 #pragma once
+#include <atomic>
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
@@ -42,6 +43,10 @@ class Server {
   bool has_disp() const { return disp_; }
   void stats(int64_t* completed, int64_t* batches, int64_t* max_inflight, int64_t* max_batch,
              int64_t* refills);
+  // device and page-locked allocations made after construction (output
+  // blocks, slot buffers that had to grow); a freed buffer synchronises the
+  // device, so a warmed-up server makes none
+  int64_t allocations() const { return allocs_.load(); }
 
  private:
   struct Slot;
@@ -65,6 +70,7 @@ class Server {
   std::unordered_map<int64_t, SrvReq*> reqs_;
   std::vector<OutBlock*> pool_;  // free output blocks
   bool stop_ = false;
+  std::atomic<int64_t> allocs_{0};
   int64_t next_id_ = 1, completed_ = 0, batches_ = 0, max_inflight_ = 0, max_batch_ = 0,
           inflight_est_ = 0, refills_ = 0, busy_ = 0;
 };

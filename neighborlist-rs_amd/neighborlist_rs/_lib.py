@@ -135,6 +135,7 @@ _SIGS = {
                                           C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "nl_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "nl_server_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p, i64p]),
+    "nl_server_allocations": (C.c_int, [C.c_void_p, i64p]),
     "nl_server_free": (None, [C.c_void_p]),
     "nl_server_create_ex": (C.c_void_p, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_uint]),
     "nl_server_take_ex": (C.c_int, [C.c_void_p, C.c_int64, i64p, i64p, i32p, dp, dp]),

@@ -141,7 +141,8 @@ class Server:
             raise RuntimeError(_lib.last_error())
 
     def stats(self):
-        v = [C.c_int64() for _ in range(5)]
-        lib.nl_server_stats(self._p, *[C.byref(x) for x in v])
-        return dict(zip(("completed", "batches", "max_inflight", "max_batch", "refills"),
-                        (int(x.value) for x in v)))
+        v = [C.c_int64() for _ in range(6)]
+        lib.nl_server_stats(self._p, *[C.byref(x) for x in v[:5]])
+        lib.nl_server_allocations(self._p, C.byref(v[5]))
+        return dict(zip(("completed", "batches", "max_inflight", "max_batch", "refills",
+                         "allocations"), (int(x.value) for x in v)))

@@ -191,7 +191,7 @@ def test_engine_setters_roundtrip(nr):
         assert lib.nl_get_bin_order() == want
     lib.nl_set_bin_order(old)
     old = lib.nl_get_fused_build()
-    for v, want in ((0, 0), (1, 1), (5, 1)):
+    for v, want in ((0, 0), (1, 1), (2, 2), (5, 1), (-1, 1)):
         lib.nl_set_fused_build(v)
         assert lib.nl_get_fused_build() == want
     lib.nl_set_fused_build(old)

@@ -406,14 +406,17 @@ def _search_ex_device(lib, L, dpos, n, sizes, H9, has, pb, cut, multi, disjoint,
 
 @pytest.mark.parametrize("tuning", [(0, 8, 2), (3, 16, 4), (2, 3, 1), (1, 64, 3)])
 def test_fused_build_identical_to_stages(nr, oracle, tuning):
-    """Batches whose systems all have at most 4096 atoms build in one launch
+    """Batches whose systems all have at most 12288 atoms build in one launch
     (k_build_systems: plan, counting sort by bin, bin offsets, sorted
-    positions, work list); the separate launches of the general path
+    positions, work list; taken for every case here with
+    nl_set_fused_build(2)); the separate launches of the general path
     (nl_set_fused_build(0)) must give the same arrays, order included:
     the 256-system ragged batch with three lists, a batch at the 4096-atom
-    boundary with isolated, orthorhombic and triclinic systems and single
-    atoms (full lists, distances, ranges), per-system cutoffs are covered by
-    the serving tests.  One case is also checked against the oracle."""
+    boundary of the segmented sort with isolated, orthorhombic and triclinic
+    systems and single atoms (full lists, distances, ranges), a batch up to
+    the 12288-atom limit, and a dense cluster with one far atom (its auto box
+    is huge, so a few bins hold every atom); per-system cutoffs are covered
+    by the serving tests.  Two cases are also checked against the oracle."""
     from neighborlist_rs import _lib as L, workloads as W
     import bench
     lib = L.lib
@@ -434,17 +437,30 @@ def test_fused_build_identical_to_stages(nr, oracle, tuning):
             ps.append(p)
             cb.append(H)
         pb_ = np.ascontiguousarray(np.concatenate(ps))
+        sizes_l = [12288, 5000, 4097, 1, 8000]  # one launch only when forced (mode 2)
+        pl, cl = [], []
+        for s, m in enumerate(sizes_l):
+            p, H = W.random_system(rng, m, (s + 1) % 3)
+            pl.append(p)
+            cl.append(H)
+        pl_ = np.ascontiguousarray(np.concatenate(pl))
+        pc = np.ascontiguousarray(np.vstack([rng.uniform(0.0, 12.0, size=(6000, 3)),
+                                             [[500.0, 500.0, 500.0]]]))
         cases = [("ragged batch", p4, np.asarray(s4, np.int64), c4, np.array(cut4), True, False,
                   0),
                  ("boundary batch, full lists, distances, ranges", pb_,
                   np.array(sizes_b, np.int64), cb, np.array([3.5, 6.0]), True, True,
-                  16 | 8 | 4)]
+                  16 | 8 | 4),
+                 ("large systems", pl_, np.array(sizes_l, np.int64), cl, np.array([5.0]), False,
+                  False, 4),
+                 ("cluster with a far atom", pc, np.array([pc.shape[0]], np.int64), [None],
+                  np.array([3.0]), False, False, 0)]
         for name, pos, sizes, cells, cut, multi, disjoint, opts in cases:
             _print_case("fused build", tuning, name)
             H9, has, pbc = bench.cell_arrays(cells)
             d = bench.DevArray(lib, pos)
             try:
-                lib.nl_set_fused_build(1)
+                lib.nl_set_fused_build(2)
                 a = _search_ex_device(lib, L, d.ptr, pos.shape[0], sizes, H9, has, pbc, cut,
                                       multi, disjoint, opts)
                 lib.nl_set_fused_build(0)
@@ -455,13 +471,13 @@ def test_fused_build_identical_to_stages(nr, oracle, tuning):
             for q in range(cut.size):
                 for x, y in zip(a[q], b[q]):
                     np.testing.assert_array_equal(x, y, err_msg=f"{name} list {q}")
-            if name == "ragged batch" and tuning[0] == 0:
+            if name in ("ragged batch", "large systems") and tuning[0] == 0:
                 batch, _ = W.batch_arrays(sizes, cells)
                 ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
                 want = oracle.search_batch_multi(pos, batch, ocells, list(cut), False)
                 for q in range(cut.size):
                     ei, sh = a[q][0], a[q][1]
-                    assert_same_edges((ei[0], ei[1], sh), want[q], f"fused list {q}")
+                    assert_same_edges((ei[0], ei[1], sh), want[q], f"{name}: fused list {q}")
     finally:
         lib.nl_set_autotune(olds[0])
         lib.nl_set_cell_div(olds[1])

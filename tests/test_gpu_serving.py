@@ -120,9 +120,11 @@ def test_serving_threads_and_budget(nr, oracle):
 
 def test_serving_refill_when_estimate_is_short(nr, oracle):
     # a dense cluster plus one far atom: the edge estimate from the bounding
-    # box is far too small, so the fill pass must run again into larger buffers
+    # box is far too small, and the cluster's ~2.3 M edges overflow the
+    # smallest output block (2^20 edges), so the fill pass must run again
+    # into a larger block
     rng = np.random.default_rng(3)
-    p = np.vstack([rng.uniform(0.0, 6.0, size=(600, 3)), [[900.0, 900.0, 900.0]]])
+    p = np.vstack([rng.uniform(0.0, 6.0, size=(2500, 3)), [[900.0, 900.0, 900.0]]])
     _print_case("serving refill", p.shape)
     with nr.Server(slots=2) as srv:
         a = srv.submit(None, p, 5.0)
@@ -220,8 +222,9 @@ def test_serving_full_lists_distances_displacements(nr, oracle):
         for x, y in zip(_sorted_all(ei, ej, sh, d, v), _sorted_all(*_torch_search(nr, p, H, rc))):
             np.testing.assert_array_equal(x, y, err_msg=what + " vs torch search")
 
+    # ~1.6 M directed edges: more than the smallest output block holds
     rng = np.random.default_rng(4)
-    p = np.vstack([rng.uniform(0.0, 6.0, size=(500, 3)), [[800.0, 800.0, 800.0]]])
+    p = np.vstack([rng.uniform(0.0, 6.0, size=(1500, 3)), [[800.0, 800.0, 800.0]]])
     _print_case("serving full+dist+disp refill", p.shape)
     with nr.Server(slots=1, **opts) as srv:
         r = srv.result(srv.submit(None, p, 5.0), 60.0)
