@@ -223,9 +223,9 @@ def serve_trace(L, n_req, slots, load):
     """Serving trace: Poisson arrivals of single systems (N log-uniform in
     [16, 50000], isolated / orthorhombic / triclinic, rc uniform in [4, 12]).
 
-    Capacity: all requests queued at once (closed loop, median of three
-    runs), results left on the device (the consumer of a served list is a GPU
-    model) and, separately, copied into numpy arrays.  Latency: open loop, Poisson arrivals at `load`
+    Capacity: all requests queued at once (closed loop, median of five runs
+    after two warm-up runs), results left on the device (the consumer of a
+    served list is a GPU model) and, separately, copied into numpy arrays.  Latency: open loop, Poisson arrivals at `load`
     times the measured capacity; latency = submit -> lists complete on the
     device.  Plain: the same requests one call each through the plain path
     (host input, lists left on the device).  Requests are submitted with
@@ -279,9 +279,14 @@ def serve_trace(L, n_req, slots, load):
                 tot += m
         return time.perf_counter() - t0, tot
 
-    closed(False)  # warm-up: grows slot buffers and the output pool
-    runs = [closed(False) for _ in range(3)]  # median of three closed loops
-    cap_s, served_edges = sorted(runs)[1]
+    # warm-up: two loops grow the slot buffers and the output pool (a warmed
+    # server allocates nothing more; a freed device buffer stalls the device)
+    closed(False)
+    closed(False)
+    allocs_warm = srv.stats()["allocations"]
+    runs = [closed(False) for _ in range(5)]  # median of five closed loops
+    allocs_timed = srv.stats()["allocations"] - allocs_warm
+    cap_s, served_edges = sorted(runs)[2]
     assert all(e == plain_edges for _, e in runs), (runs, plain_edges)
     host_s, host_edges = closed(True)
     assert host_edges == plain_edges
@@ -341,6 +346,7 @@ def serve_trace(L, n_req, slots, load):
                       "batches": st["batches"] - st0["batches"],
                       "max_inflight": st["max_inflight"]},
         "refills": st["refills"],
+        "allocations": {"warm_up": allocs_warm, "timed_closed_loops": allocs_timed},
     }
 
 

@@ -1745,24 +1745,41 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ key
 //
 // Blocks are dispatched round-robin over the 8 XCDs (block b to XCD b % 8),
 // each with its own L2.  With xcd_chunk = c > 0 the work list is dealt to
-// the XCDs in runs of c blocks (XCD x takes runs x, x + 8, ...), so
+// the XCDs in runs of 4c items (XCD x takes runs x, x + 8, ...), so
 // neighbouring home groups, which read the same stencil bins, run under one
 // L2 while every XCD still sees the whole mix of long and short items;
 // c = 0 keeps the dispatch order.  Only the blocks that have an item take
 // part; the others leave at once.
+//
+// Fill blocks of several lists hold eight waves, other blocks four: the item
+// slots past the work list take a block each way, and dispatching the
+// ragged batch's 85,000 four-wave blocks costs ~19 us per launch.  On the
+// ragged batch (three lists) eight-wave fill blocks measured 10 us faster
+// per fill and eight-wave count blocks 14 us slower (a block keeps its
+// slots until its slowest wave ends); on the serving trace (one list per
+// system) eight-wave fill blocks cost 5 % of the capacity.  The XCD runs
+// keep their length in items (xcd_chunk counts four-wave blocks).
+constexpr int kWavesFill = 8;
+template <bool FILL, int NQ>
+constexpr int waves_per_block() {
+  return FILL && NQ > 1 ? kWavesFill : kWaves;
+}
+
 template <bool FILL, int NQ, bool F32, bool FULL, bool DIST, bool DISP>
 __device__ __forceinline__ void pairs_body(const PairArgs& A) {
   using Lds = WaveLds<F32, FILL>;
-  __shared__ Lds s_lds[kWaves];
+  constexpr int WPB = waves_per_block<FILL, NQ>();
+  __shared__ Lds s_lds[WPB];
   const int w = uniform_i32(threadIdx.x >> 6);
   const int64_t front = A.n_real[0];
   const int64_t total = front + (int64_t)A.n_real[1];
-  const int64_t nblk = (total + kWaves - 1) / kWaves;
+  const int64_t nblk = (total + WPB - 1) / WPB;
   const int64_t b = blockIdx.x;
   if (b >= nblk) return;
   constexpr int64_t kXcd = 8;
   int64_t bb = b;
-  const int64_t ch = A.xcd_chunk;
+  const int64_t ch0 = (int64_t)A.xcd_chunk * kWaves / WPB;  // blocks per run
+  const int64_t ch = A.xcd_chunk > 0 ? (ch0 > 0 ? ch0 : 1) : 0;
   if (ch > 0) {
     const int64_t span = kXcd * ch;
     if (b < nblk / span * span) {  // the tail past whole spans keeps its order
@@ -1770,7 +1787,7 @@ __device__ __forceinline__ void pairs_body(const PairArgs& A) {
       bb = (q / ch) * span + (b % kXcd) * ch + q % ch;
     }
   }
-  const int64_t k = bb * kWaves + w;
+  const int64_t k = bb * WPB + w;
   if (k >= total) return;
   // wave-uniform item: its loads are scalar and the fill cursors stay scalar
   const int64_t u = uniform_i64(k < front ? A.list[k] : A.list[A.n - 1 - (k - front)]);
@@ -1781,7 +1798,7 @@ __device__ __forceinline__ void pairs_body(const PairArgs& A) {
 }
 
 template <bool FILL, int NQ, bool F32, bool FULL, bool DIST = false, bool DISP = false>
-__global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
+__global__ __launch_bounds__((64 * waves_per_block<FILL, NQ>())) void k_pairs(PairArgs A) {
   pairs_body<FILL, NQ, F32, FULL, DIST, DISP>(A);
 }
 
@@ -1789,7 +1806,8 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs A) {
 // path without the full-list fill and without distances or displacements,
 // where that fits without scratch and measured a few per cent faster.
 template <bool FILL, int NQ, bool F32, bool FULL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_w6(
+__global__ __launch_bounds__((64 * waves_per_block<FILL, NQ>())) __attribute__((amdgpu_waves_per_eu(6)))
+void k_pairs_w6(
     PairArgs A) {
   pairs_body<FILL, NQ, F32, FULL, false, false>(A);
 }
@@ -2507,14 +2525,16 @@ static void fill_full(const dim3& g, const dim3& b, const PairArgs& a, hipStream
 
 template <int NQ>
 static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
-  const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
   if (fill) {
+    constexpr int wf = waves_per_block<true, NQ>();
+    const dim3 g(grid_for(a.n, wf)), b(64 * wf);
     if (a.use_fp32)
       fill_full<NQ, true>(g, b, a, st);
     else
       fill_full<NQ, false>(g, b, a, st);
     return;
   }
+  const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
   if (a.use_fp32) {
     if (a.full)
       hipLaunchKernelGGL((k_pairs<false, NQ, true, true>), g, b, 0, st, a);
