@@ -314,6 +314,9 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.n_real = b.zero + 2 * P.stiles + 2 * P.rstiles * P.passes + 6;
   A.bounds = b.bounds;
   A.n = P.n_items;
+  // at most one item per atom (rows) or per slot and stencil slice (home
+  // groups start at distinct slots): the pair-kernel grids stop there
+  A.max_items = P.n * std::max(1, P.tune.parts);
   A.n_sys = P.S;
   A.k = P.K;
   A.disjoint = P.disjoint;

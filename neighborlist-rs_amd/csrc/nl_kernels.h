@@ -33,6 +33,7 @@ struct PairArgs {
   double* dist;              // optional per edge: d^2 from the fill pass, then |r_ij|
   double* disp;              // optional per edge: r_ij = r_j + H S - r_i, 3 doubles
   int64_t n;                 // work items
+  int64_t max_items;         // bound on the items with work (the launch grid)
   int64_t capacity;          // edges the output buffers hold (fill pass)
   int32_t n_sys, k, q_lo, q_cnt, disjoint, use_fp32, global_index;
   int32_t per_sys_rc;        // rc2 is [n_sys][k] (per-system cutoffs)

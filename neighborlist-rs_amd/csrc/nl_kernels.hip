@@ -2525,16 +2525,19 @@ static void fill_full(const dim3& g, const dim3& b, const PairArgs& a, hipStream
 
 template <int NQ>
 static void launch_pairs_nq(bool fill, const PairArgs& a, hipStream_t st) {
+  // one wave per item that can have work (the blocks past the work list
+  // leave at once, but each costs dispatch time)
+  const int64_t items = a.max_items > 0 ? std::min(a.max_items, a.n) : a.n;
   if (fill) {
     constexpr int wf = waves_per_block<true, NQ>();
-    const dim3 g(grid_for(a.n, wf)), b(64 * wf);
+    const dim3 g(grid_for(items, wf)), b(64 * wf);
     if (a.use_fp32)
       fill_full<NQ, true>(g, b, a, st);
     else
       fill_full<NQ, false>(g, b, a, st);
     return;
   }
-  const dim3 g(grid_for(a.n, kWaves)), b(64 * kWaves);
+  const dim3 g(grid_for(items, kWaves)), b(64 * kWaves);
   if (a.use_fp32) {
     if (a.full)
       hipLaunchKernelGGL((k_pairs<false, NQ, true, true>), g, b, 0, st, a);
