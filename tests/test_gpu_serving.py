@@ -294,3 +294,27 @@ def test_serving_borrowed_positions(nr, oracle):
     _, p, H, rc = reqs[0]
     want = oracle.search_single(p, H, None if H is None else [True] * 3, rc)
     assert_same_edges(as_tuple(borrowed[0]), want, "request 0")
+
+
+def test_serving_warm_server_stops_allocating(nr):
+    """Slot buffers grow towards the largest admitted batch and output blocks
+    come in capacity classes, so repeating a trace reaches a pass with no
+    new device or page-locked allocation (a freed device buffer would
+    synchronise the device); every pass returns the same arrays."""
+    reqs = _trace(seed=31, n=30)
+    _print_case("serving warm server", len(reqs))
+    with nr.Server(slots=2, max_atoms=20000, max_systems=64) as srv:
+        first, counts = None, [srv.stats()["allocations"]]
+        for _ in range(6):
+            ids = [srv.submit(_cell(nr, H), p, rc) for (_, p, H, rc) in reqs]
+            got = [srv.result(rid, timeout=60.0) for rid in ids]
+            counts.append(srv.stats()["allocations"])
+            if first is None:
+                first = got
+            else:
+                for k, (a, b) in enumerate(zip(first, got)):
+                    np.testing.assert_array_equal(a["edge_index"], b["edge_index"],
+                                                  err_msg=f"request {k}")
+                    np.testing.assert_array_equal(a["shift"], b["shift"], err_msg=f"request {k}")
+    assert counts[1] > counts[0], counts
+    assert any(b == a for a, b in zip(counts[1:], counts[2:])), counts
