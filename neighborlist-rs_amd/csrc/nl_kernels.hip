@@ -2321,15 +2321,30 @@ __global__ __launch_bounds__(1024) void k_scan_sums_excl(uint64_t* __restrict__ 
 
 // Tile `blockIdx.x` of the scan, starting from its offset pre[tile].
 // bounds[q] = out[q * seg] for q < nseg; bounds[nseg] = out[len] = total.
+// RAW: pre holds the tile sums themselves, and the block adds up those
+// before its tile (up to kScanRawTiles tiles: one launch fewer than the
+// separate scan of the sums).
+constexpr int64_t kScanRawTiles = 2048;
+template <bool RAW>
 __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in,
                                                     int64_t* __restrict__ out, int64_t len,
                                                     int64_t seg, int nseg,
                                                     int64_t* __restrict__ bounds,
                                                     const uint64_t* __restrict__ pre) {
   __shared__ uint64_t s_w[4];
+  __shared__ uint64_t s_pre;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
   const int64_t e0 = t0 + 8 * tid;
+  if constexpr (RAW) {
+    uint64_t part = 0;
+    for (int64_t t = tid; t < (int64_t)blockIdx.x; t += 256) part += pre[t];
+    part = wave_sum_u64(part);
+    if (lane == 0) s_w[w] = part;
+    __syncthreads();
+    if (tid == 0) s_pre = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+  }
   uint64_t v[8];
   uint64_t sum = 0;
 #pragma unroll
@@ -2340,7 +2355,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
   const uint64_t winc = wave_incl_scan(sum);
   if (lane == 63) s_w[w] = winc;
   __syncthreads();
-  uint64_t run = pre[blockIdx.x] + winc - sum;
+  uint64_t run = (RAW ? s_pre : pre[blockIdx.x]) + winc - sum;
   for (int i = 0; i < w; ++i) run += s_w[i];
   // list bounds that fall in this tile (block-uniform range of q)
   const int64_t t1 = min(t0 + (int64_t)kRadixTile, len);
@@ -2706,11 +2721,17 @@ hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t se
                        bounds, status, ticket);
     return hipGetLastError();
   }
-  // status (one u64 per tile) holds the tile sums, then their prefixes
+  // status (one u64 per tile) holds the tile sums, then (longer scans)
+  // their prefixes
   hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)tiles), dim3(256), 0, st, in, len, status);
+  if (tiles <= kScanRawTiles) {
+    hipLaunchKernelGGL(k_scan_apply<true>, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len,
+                       seg, nseg, bounds, (const uint64_t*)status);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_scan_sums_excl, dim3(1), dim3(1024), 0, st, status, tiles);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len, seg,
-                     nseg, bounds, (const uint64_t*)status);
+  hipLaunchKernelGGL(k_scan_apply<false>, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len,
+                     seg, nseg, bounds, (const uint64_t*)status);
   return hipGetLastError();
 }
 
