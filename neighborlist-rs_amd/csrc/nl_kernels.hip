@@ -1410,8 +1410,12 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
         int sg = 0, j = 0;
         bool valid = false;
         float4 n32 = make_float4(0.f, 0.f, 0.f, 0.f);
-        Pos64 n64 = {0.0, 0.0, 0.0, 0.0};
-        int4 nij = make_int4(0, 0, 0, 0);
+        // the candidate's coordinates and local index (7 of the 8 words of
+        // its Pos64) and wrap shift (3 of 4): fewer registers held across
+        // the chunk for the next one
+        double3 n64 = make_double3(0.0, 0.0, 0.0);
+        int n64i = 0;
+        int3 nij = make_int3(0, 0, 0);
         // This lane's candidate of the chunk at c0 lies in the last
         // non-empty segment starting at or before it.  Lane s holds segment
         // s.  The segment holding candidate c0 is one ballot; each segment
@@ -1443,11 +1447,17 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             // loads (a flat load also counts against the LDS wait of every
             // step of the home-atom loop, which would expose its latency)
             const uint64_t uj = (uint32_t)j;
-            if (F32)
+            if (F32) {
               n32 = A.spos32[uj];
-            else
-              n64 = A.spos64[uj];
-            if (FILL) nij = A.sidx[uj];
+            } else {
+              const Pos64* q = A.spos64 + uj;
+              n64 = make_double3(q->x, q->y, q->z);
+              n64i = reinterpret_cast<const int*>(q)[6];  // low word of w
+            }
+            if (FILL) {
+              const int* t = reinterpret_cast<const int*>(A.sidx + uj);
+              nij = make_int3(t[0], t[1], t[2]);
+            }
           }
         };
         locate(0);
@@ -1456,8 +1466,9 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
           const int csg = sg, cjs = j;
           const bool cvalid = valid;
           const float4 c32 = n32;
-          Pos64 c = n64;
-          const int4 cj = nij;
+          double3 c = n64;
+          const int ci = n64i;
+          const int3 cj = nij;
           if (c0 + 64 < total) {
             locate(c0 + 64);
             fetch();
@@ -1479,7 +1490,7 @@ __device__ __forceinline__ void unit_cell(const PairArgs& A, const Cuts<NQ>& C, 
             c.x = NAN;
             px = NAN;
           }
-          const int oj = F32 ? __float_as_int(c32.w) : __double2loint(c.w);
+          const int oj = F32 ? __float_as_int(c32.w) : ci;
           // fill pass, per chunk: the candidate's part of S (s_j + bs) and its
           // emitted index
           const int tjx = cj.x + sbx, tjy = cj.y + sby, tjz = cj.z + sbz;
