@@ -30,7 +30,8 @@ struct Config {
   // block up to 128 atoms, two launches up to 1024), 2 one block and 3 two
   // launches for every single system up to 1024 atoms (the plan decides)
   std::atomic<int> small_path{1};
-  // pair kernels: work-list blocks per XCD run (0: dispatch order);
+  // pair kernels: XCD runs of this many work-list items in the count pass,
+  // four times as many in the fill (0: dispatch order);
   // NLMI355_XCD_CHUNK overrides the default
   std::atomic<int> xcd_chunk{16};
   // cell-list bins keyed by their linear index (0) or their Morton rank (1);

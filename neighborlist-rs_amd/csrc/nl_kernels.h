@@ -40,7 +40,7 @@ struct PairArgs {
   int32_t full;              // full lists: every pair in both directions
   int32_t group;             // home atoms per cell-list work item (<= 64)
   int32_t parts;             // slices of the half stencil per home group
-  int32_t xcd_chunk;         // work-list blocks per XCD run (0: dispatch order)
+  int32_t xcd_chunk;         // XCD run length: count items (fill: 4x), 0: dispatch order
 };
 
 // One single system of at most kSmallRows atoms for k_small: its descriptor
