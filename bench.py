@@ -431,14 +431,15 @@ def _median_ms(fn, reps, warm=2):
     return 1e3 * float(np.median(tt))
 
 
-def config_lines(lib, L, O, threads):
+def config_lines(lib, L, O, threads, only=None):
     """The benchmark configurations one by one, and the reference's own
     published cases, each with: GPU time with positions and lists on the
     device (host clock per call, median), the device stages alone (hipEvents),
     numpy in / numpy out through the Python API (host copies included, as
     the reference's published times), the roofline fraction of the device
     stages (24 B per atom + 72 B per cell + 28 B per edge against 8 TB/s), and
-    the oracle on `threads` host cores (None: no CPU timing)."""
+    the oracle on `threads` host cores (None: no CPU timing).  only: the
+    name of the one line to run (all when None)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from xyzio import read_xyz
     import neighborlist_rs as nr
@@ -446,6 +447,8 @@ def config_lines(lib, L, O, threads):
     out = {}
 
     def single(name, pos, H, pbc, cutoffs, published=None, reps=30):
+        if only is not None and name != only:
+            return
         n = pos.shape[0]
         multi = len(cutoffs) > 1
         d = DevArray(lib, np.ascontiguousarray(pos))
@@ -519,6 +522,8 @@ def config_lines(lib, L, O, threads):
 
     # the reference's GNN batch (batch_radius_benchmark.py:109-133,
     # batch_results.json:2-12): numpy in, numpy out, no cells
+    if only is not None and only != "gnn_molecule_batch_rc6":
+        return out
     pg, bg = W.gnn_molecule_batch()
     res = [None]
 
@@ -621,6 +626,8 @@ def main():
                     help="run only the timed batch steps (for rocprofv3)")
     ap.add_argument("--only-configs", action="store_true",
                     help="run only the per-configuration lines")
+    ap.add_argument("--only-config", default=None,
+                    help="with --only-configs: the one line to run (e.g. c3_triclinic_10008_rc14)")
     ap.add_argument("--bin-order", type=int, default=-1,
                     help="1: Morton-ranked bins, 0: linear bin order (the engine default)")
     ap.add_argument("--serve-requests", type=int, default=600,
@@ -669,7 +676,8 @@ def main():
         if not args.no_cpu:
             from oracle import oracle as O
         threads = O.max_threads() if O is not None else None
-        print(json.dumps({"configs": config_lines(lib, L, O, threads)}), flush=True)
+        print(json.dumps({"configs": config_lines(lib, L, O, threads, args.only_config)}),
+              flush=True)
         return
 
     if args.only_serving:
