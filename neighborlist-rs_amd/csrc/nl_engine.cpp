@@ -752,7 +752,10 @@ int Engine::ensure_small_buf() {
     set_error("out of device memory");
     return 1;
   }
-  NL_CHECK(hipMemset(small_buf_, 0, kSmBytes));  // the tickets start at zero
+  // the tickets start at zero (on the engine's stream: a blocking hipMemset
+  // would put the null stream on one of the runtime's hardware queues)
+  NL_CHECK(hipMemsetAsync(small_buf_, 0, kSmBytes, stream_));
+  NL_CHECK(hipStreamSynchronize(stream_));
   return 0;
 }
 

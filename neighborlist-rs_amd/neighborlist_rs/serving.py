@@ -30,6 +30,12 @@ class Server:
     edge_budget  estimated edges of all in-flight batches (admission control;
                  a request alone is always admitted)
     full, distances, displacements   outputs of every request (see above)
+
+    Each slot's stream should get a hardware queue of its own: HIP's default
+    of four (GPU_MAX_HW_QUEUES, read when the runtime initialises) is shared
+    with the engine's stream and the null stream, and two slots on one queue
+    cost 6 % of the capacity with three slots.  Set GPU_MAX_HW_QUEUES to at
+    least slots + 2 before the process first touches the GPU.
     """
 
     def __init__(self, slots=3, max_atoms=1 << 20, max_systems=1024, edge_budget=1 << 28, *,
