@@ -330,13 +330,19 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   return A;
 }
 
-hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const {
+static_assert(sizeof(SysDesc) % 8 == 0, "the upload block is copied in whole words");
+hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st,
+                                         const void* restore) const {
   uint64_t* sstatus = (uint64_t*)b.zero;
   uint32_t* tickets = b.zero + 2 * stiles + 2 * rstiles * passes;
+  // the upload block is whole 8-byte words (descriptors, i64 and f64 arrays)
+  uint32_t* cdst = restore ? (uint32_t*)b.sys : nullptr;
+  const uint32_t* csrc = (const uint32_t*)restore;
+  const int64_t cwords = restore ? (int64_t)(off_bounds() / 4) : 0;
   if (fused_build) {
     // one launch builds every system (one block each): plan, sort, bin
     // offsets, sorted positions, work list, zero counts of idle items
-    NL_TRY(launch_zero2(b.zero, z_words, nullptr, 0, st));
+    NL_TRY(launch_zero2(b.zero, z_words, nullptr, 0, st, cdst, csrc, cwords));
     BuildArgs ba{};
     ba.pos = b.pos;
     ba.sys = b.sys;
@@ -359,7 +365,7 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st) const
     NL_TRY(launch_build_systems(ba, S, st));
   } else {
     // zero block and counts (items without work keep zero counts)
-    NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st));
+    NL_TRY(launch_zero2(b.zero, z_words, b.counts, scan_len, st, cdst, csrc, cwords));
     NL_TRY(enqueue_build_stages(b, st));
   }
   PairArgs A = pair_args(*this, b);
@@ -1276,8 +1282,10 @@ int Plan::capture() {
   }
   hipError_t e = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return eng_->fail(e, "hipStreamBeginCapture");
-  e = pipe_.enqueue_upload(buf_, pinned_, st_);
-  if (e == hipSuccess) e = pipe_.enqueue_build_count(buf_, st_);
+  // positions are device-resident and the upload block never changes: the
+  // first kernel restores the descriptors (which the build overwrites with
+  // its plans) from the plan's device copy
+  e = pipe_.enqueue_build_count(buf_, st_, d_static_);
   if (e == hipSuccess) e = pipe_.enqueue_fill(buf_, edges_, shifts_, dist_, cap_, st_, disp_);
   if (e == hipSuccess) e = pipe_.enqueue_bounds(buf_, pinned_, st_);
   if (e == hipSuccess && want_ranges_) {
@@ -1346,6 +1354,11 @@ int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
     return 1;
   }
   pipe_.stage_host(pinned_);
+  if (hipMalloc(&d_static_, std::max<size_t>(pipe_.off_bounds(), 16)) != hipSuccess) {
+    set_error("out of device memory for the plan");
+    return 1;
+  }
+  owned_.push_back(d_static_);
   if (want_ranges_) {
     const size_t rb = 16 * (size_t)pipe_.S * pipe_.K;
     if (hipMalloc((void**)&d_ranges_, std::max<size_t>(rb, 16)) != hipSuccess) {
@@ -1363,6 +1376,12 @@ int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
   if (hipStreamCreateWithFlags(&st_, hipStreamNonBlocking) != hipSuccess) {
     set_error("stream creation failed");
     return 1;
+  }
+  {
+    hipError_t e = hipMemcpyAsync(d_static_, pinned_, pipe_.off_bounds(),
+                                  hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return eng_->fail(e, "plan upload");
   }
   if (capacity <= 0) {
     // size the lists from one eager count

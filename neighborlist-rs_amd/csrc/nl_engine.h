@@ -125,7 +125,11 @@ class Pipeline {
 
   // Stages, all asynchronous on `st`.
   hipError_t enqueue_upload(const Buffers& b, const char* pinned, hipStream_t st) const;
-  hipError_t enqueue_build_count(const Buffers& b, hipStream_t st) const;
+  // restore: a device copy of the upload block (descriptors, system starts,
+  // item bases, cutoffs), copied in by the first launch in place of
+  // enqueue_upload's host copy (graph-captured plans: one node fewer)
+  hipError_t enqueue_build_count(const Buffers& b, hipStream_t st,
+                                 const void* restore = nullptr) const;
   hipError_t enqueue_build_stages(const Buffers& b, hipStream_t st) const;
   hipError_t enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
                           int64_t capacity, hipStream_t st, void* disp = nullptr) const;
@@ -331,6 +335,7 @@ class Plan {
   Buffers buf_;
   std::vector<void*> owned_;
   char* pinned_ = nullptr;
+  void* d_static_ = nullptr;  // device copy of the upload block (set at create)
   hipStream_t st_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;

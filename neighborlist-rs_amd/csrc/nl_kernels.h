@@ -127,7 +127,9 @@ hipError_t launch_small(const SmallArgs& a, hipStream_t st);
 // count launch (fill = false; writes bounds), then fill launch (fill = true)
 hipError_t launch_brute(bool fill, const SmallArgs& a, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
-hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st);
+// clears a[0, na) and b[0, nb), and copies src[0, nc) to dst (u32 words)
+hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st,
+                        uint32_t* dst = nullptr, const uint32_t* src = nullptr, int64_t nc = 0);
 hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity,
                             hipStream_t st);
 hipError_t launch_plan(const double* pos, SysDesc* sys, float2* band, const double* rc2,

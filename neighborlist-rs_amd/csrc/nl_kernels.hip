@@ -161,15 +161,20 @@ __global__ void k_sqrt_dist(double* __restrict__ dist, const int64_t* __restrict
 }
 
 // Clears two word ranges in one launch (a kernel, not a memset node, so the
-// launch sequence stays safe to capture in a graph).
+// launch sequence stays safe to capture in a graph), and copies nc words
+// from src to dst (a range apart from both) in the same launch.
 __global__ void k_zero2(uint32_t* __restrict__ a, int64_t na, uint32_t* __restrict__ b,
-                        int64_t nb) {
+                        int64_t nb, uint32_t* __restrict__ dst,
+                        const uint32_t* __restrict__ src, int64_t nc) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb + nc;
+       i += stride) {
     if (i < na)
       a[i] = 0u;
-    else
+    else if (i < na + nb)
       b[i - na] = 0u;
+    else
+      dst[i - na - nb] = src[i - na - nb];
   }
 }
 
@@ -2455,11 +2460,13 @@ hipError_t launch_sqrt_dist(double* dist, const int64_t* total, int64_t capacity
   return hipGetLastError();
 }
 
-hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st) {
-  if (na + nb <= 0) return hipSuccess;
-  int64_t g = (na + nb + 255) / 256;
+hipError_t launch_zero2(uint32_t* a, int64_t na, uint32_t* b, int64_t nb, hipStream_t st,
+                        uint32_t* dst, const uint32_t* src, int64_t nc) {
+  if (!dst || !src) nc = 0;
+  if (na + nb + nc <= 0) return hipSuccess;
+  int64_t g = (na + nb + nc + 255) / 256;
   if (g > 4096) g = 4096;
-  hipLaunchKernelGGL(k_zero2, dim3((unsigned)g), dim3(256), 0, st, a, na, b, nb);
+  hipLaunchKernelGGL(k_zero2, dim3((unsigned)g), dim3(256), 0, st, a, na, b, nb, dst, src, nc);
   return hipGetLastError();
 }
 
