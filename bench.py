@@ -642,7 +642,7 @@ def main():
                     help="1: Morton-ranked bins, 0: linear bin order (the engine default)")
     ap.add_argument("--serve-requests", type=int, default=600,
                     help="requests in the serving trace (0: skip serving)")
-    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="open-loop arrival rate as a fraction of the served capacity")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
