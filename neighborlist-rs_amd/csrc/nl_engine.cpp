@@ -341,8 +341,12 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st,
   const int64_t cwords = restore ? (int64_t)(off_bounds() / 4) : 0;
   if (fused_build) {
     // one launch builds every system (one block each): plan, sort, bin
-    // offsets, sorted positions, work list, zero counts of idle items
-    NL_TRY(launch_zero2(b.zero, z_words, nullptr, 0, st, cdst, csrc, cwords));
+    // offsets, sorted positions, work list, zero counts of idle items.
+    // With `restore` (a captured plan) it plans from the device copy of the
+    // descriptors and clears the scan's words itself: no launch before it.
+    // The work list's lengths are zero then already (the scan of the
+    // previous replay cleared them, or the plan's set-up did).
+    if (!restore) NL_TRY(launch_zero2(b.zero, z_words, nullptr, 0, st));
     BuildArgs ba{};
     ba.pos = b.pos;
     ba.sys = b.sys;
@@ -362,6 +366,9 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st,
     ba.list = b.list;
     ba.n_real = tickets + 6;
     ba.counts = b.counts;
+    ba.sys_in = static_cast<const SysDesc*>(restore);
+    ba.zero = restore ? b.zero : nullptr;
+    ba.n_zero = restore ? (int64_t)(tickets + 6 - b.zero) : 0;
     NL_TRY(launch_build_systems(ba, S, st));
   } else {
     // zero block and counts (items without work keep zero counts)
@@ -374,8 +381,10 @@ hipError_t Pipeline::enqueue_build_count(const Buffers& b, hipStream_t st,
     A.q_cnt = std::min(kGroup, K - q0);
     NL_TRY(launch_pairs(false, A, st));
   }
+  // the scan also hands the work list's lengths on to the fill (tickets
+  // 8-9) and clears them (tickets 6-7)
   NL_TRY(launch_scan(b.counts, b.offsets, scan_len, n_items, K, b.bounds, sstatus, tickets + 4,
-                     st));
+                     st, tickets + 6));
   return hipSuccess;
 }
 
@@ -412,6 +421,7 @@ hipError_t Pipeline::enqueue_build_stages(const Buffers& b, hipStream_t st) cons
 hipError_t Pipeline::enqueue_fill(const Buffers& b, void* edges, void* shifts, void* dist,
                                   int64_t capacity, hipStream_t st, void* disp) const {
   PairArgs A = pair_args(*this, b);
+  A.n_real += 2;  // the lengths as the count scan handed them on
   A.edges = (int64_t*)edges;
   A.shifts = (int32_t*)shifts;
   A.dist = (double*)dist;
@@ -1378,8 +1388,14 @@ int Plan::create(Engine* eng, const Request& req, int64_t capacity) {
     return 1;
   }
   {
+    // the device copy, the block in place (system starts, item bases and
+    // cutoffs are only read: a replay restores the descriptors alone) and a
+    // cleared zero block (a replay's one-launch build expects the work
+    // list's lengths at zero)
     hipError_t e = hipMemcpyAsync(d_static_, pinned_, pipe_.off_bounds(),
                                   hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess) e = pipe_.enqueue_upload(buf_, pinned_, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(buf_.zero, 0, 4 * (size_t)pipe_.z_words, st_);
     if (e == hipSuccess) e = hipStreamSynchronize(st_);
     if (e != hipSuccess) return eng_->fail(e, "plan upload");
   }

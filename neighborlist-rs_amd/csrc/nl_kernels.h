@@ -117,8 +117,11 @@ struct BuildArgs {
   float4* spos32;
   int4* sidx;
   int64_t* list;             // work list: group items from the front, rows from the back
-  uint32_t* n_real;          // [2] lengths of both ends
+  uint32_t* n_real;          // [2] lengths of both ends (zero before the launch)
   uint32_t* counts;          // [k][n_items]
+  const SysDesc* sys_in;     // descriptors to plan from (null: sys itself)
+  uint32_t* zero;            // words cleared by the launch (n_zero of them; not n_real)
+  int64_t n_zero;
 };
 hipError_t launch_build_systems(const BuildArgs& a, int n_sys, hipStream_t st);
 
@@ -156,8 +159,11 @@ hipError_t launch_items(const uint32_t* keys, const int32_t* cell_start, const S
                         const int64_t* item_base, int n_sys, int64_t n_items, int group,
                         int parts, int64_t* list, uint32_t* count, hipStream_t st);
 hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st);
+// rearm (or null): the work list's lengths [2], handed on to rearm[2..3] for
+// the fill pass and cleared for the next build
 hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
-                       int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st);
+                       int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st,
+                       uint32_t* rearm = nullptr);
 // *out = bits of max_a |pos_a - ref_a|^2 (cleared first, in the same stream)
 hipError_t launch_max_disp2(const double* pos, const double* ref, int64_t n,
                             unsigned long long* out, hipStream_t st);

@@ -760,7 +760,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_systems(BuildArgs a) {
   const int s = blockIdx.x;
   const int k = a.prm.k;
   const double* rc2s = a.prm.per_sys_rc ? a.rc2 + (int64_t)s * k : a.rc2;
-  if (tid == 0) s_d = a.sys[s];
+  // words the later launches need cleared (when no clearing launch ran)
+  for (int64_t i = (int64_t)s * T + tid; i < a.n_zero; i += (int64_t)gridDim.x * T) a.zero[i] = 0u;
+  if (tid == 0) s_d = a.sys_in ? a.sys_in[s] : a.sys[s];
   __syncthreads();
   plan_system(a.pos, &s_d, a.band + (int64_t)s * k, rc2s, a.prm);
   __syncthreads();
@@ -2204,17 +2206,33 @@ __global__ __launch_bounds__(64 * kBatchRows) void k_batch_brute_fill(BatchSmall
 }
 
 // ---------------------------------------------------------------------------
+// The count scan runs after the count pass, the last reader of the work
+// list's lengths n_real = rearm[0..1]: block 0 hands them on to the fill
+// pass in rearm[2..3] and clears them for the next build (a graph replay
+// then needs no clearing launch before its build).
+__device__ __forceinline__ void rearm_list(uint32_t* rearm) {
+  if (rearm && blockIdx.x == 0 && threadIdx.x == 0) {
+    rearm[2] = rearm[0];
+    rearm[3] = rearm[1];
+    rearm[0] = 0u;
+    rearm[1] = 0u;
+  }
+}
+__global__ void k_rearm(uint32_t* rearm) { rearm_list(rearm); }
+
 // Exclusive scan of u32 counts into int64 offsets, decoupled look-back.
 // bounds[q] = offset of element q*seg (q < nseg), bounds[nseg] = total.
 __global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
                                               int64_t* __restrict__ out, int64_t len,
                                               int64_t seg, int nseg, int64_t* __restrict__ bounds,
                                               uint64_t* __restrict__ status,
-                                              uint32_t* __restrict__ ticket) {
+                                              uint32_t* __restrict__ ticket,
+                                              uint32_t* __restrict__ rearm) {
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_w[4];
   __shared__ uint64_t s_excl;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  rearm_list(rearm);
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
@@ -2348,10 +2366,12 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
                                                     int64_t* __restrict__ out, int64_t len,
                                                     int64_t seg, int nseg,
                                                     int64_t* __restrict__ bounds,
-                                                    const uint64_t* __restrict__ pre) {
+                                                    const uint64_t* __restrict__ pre,
+                                                    uint32_t* __restrict__ rearm) {
   __shared__ uint64_t s_w[4];
   __shared__ uint64_t s_pre;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  rearm_list(rearm);
   const int64_t t0 = (int64_t)blockIdx.x * kRadixTile;
   const int64_t e0 = t0 + 8 * tid;
   if constexpr (RAW) {
@@ -2733,12 +2753,16 @@ hipError_t launch_pairs(bool fill, const PairArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t seg, int nseg,
-                       int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st) {
-  if (len <= 0) return hipSuccess;
+                       int64_t* bounds, uint64_t* status, uint32_t* ticket, hipStream_t st,
+                       uint32_t* rearm) {
+  if (len <= 0) {
+    if (rearm) hipLaunchKernelGGL(k_rearm, dim3(1), dim3(1), 0, st, rearm);
+    return hipGetLastError();
+  }
   const int64_t tiles = radix_tiles(len);
   if (tiles <= kScanChainTiles) {
     hipLaunchKernelGGL(k_scan, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len, seg, nseg,
-                       bounds, status, ticket);
+                       bounds, status, ticket, rearm);
     return hipGetLastError();
   }
   // status (one u64 per tile) holds the tile sums, then (longer scans)
@@ -2746,12 +2770,12 @@ hipError_t launch_scan(const uint32_t* in, int64_t* out, int64_t len, int64_t se
   hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)tiles), dim3(256), 0, st, in, len, status);
   if (tiles <= kScanRawTiles) {
     hipLaunchKernelGGL(k_scan_apply<true>, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len,
-                       seg, nseg, bounds, (const uint64_t*)status);
+                       seg, nseg, bounds, (const uint64_t*)status, rearm);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_scan_sums_excl, dim3(1), dim3(1024), 0, st, status, tiles);
   hipLaunchKernelGGL(k_scan_apply<false>, dim3((unsigned)tiles), dim3(256), 0, st, in, out, len,
-                     seg, nseg, bounds, (const uint64_t*)status);
+                     seg, nseg, bounds, (const uint64_t*)status, rearm);
   return hipGetLastError();
 }
 
