@@ -189,6 +189,20 @@ int Pipeline::prepare(const Request& req, const Tuning& t) {
   // work items: per system its atoms (brute-force rows), then
   // (slot, stencil slice) pairs (cell-list home groups)
   n_items = n * (1 + tune.parts);
+  // Items that can hold work, at most (the pair-kernel grids stop there): a
+  // system has n_s rows on the brute-force path, or `parts` items per home
+  // group on the cell list; groups start every `group` slots of a bin, so
+  // there are at most ceil(n_s / group) + min(n_s, bins) of them, and at
+  // most n_s (one per slot)
+  {
+    const int64_t G = std::max(1, tune.group), Pp = std::max(1, tune.parts);
+    max_work = 0;
+    for (int s = 0; s < S; ++s) {
+      const int64_t ns = desc[s].n_atoms;
+      const int64_t groups = std::min(ns, (ns + G - 1) / G + std::min<int64_t>(ns, desc[s].bin_cap));
+      max_work += std::max(ns, Pp * groups);
+    }
+  }
   starts = req.starts;
   item_base.resize(S + 1);
   for (int s = 0; s <= S; ++s) item_base[s] = starts[s] * (1 + tune.parts);
@@ -314,9 +328,8 @@ static PairArgs pair_args(const Pipeline& P, const Buffers& b) {
   A.n_real = b.zero + 2 * P.stiles + 2 * P.rstiles * P.passes + 6;
   A.bounds = b.bounds;
   A.n = P.n_items;
-  // at most one item per atom (rows) or per slot and stencil slice (home
-  // groups start at distinct slots): the pair-kernel grids stop there
-  A.max_items = P.n * std::max(1, P.tune.parts);
+  // the bound on the items with work: the pair-kernel grids stop there
+  A.max_items = P.max_work;
   A.n_sys = P.S;
   A.k = P.K;
   A.disjoint = P.disjoint;

@@ -137,6 +137,7 @@ class Pipeline {
   const int64_t* host_bounds(const char* pinned) const;
   int rank_of(int list) const;  // input list -> sorted rank
 
+  int64_t max_work = 0;  // bound on the work items with work (pair-kernel grids)
   int64_t n = 0, n_items = 0, bin_total = 0, scan_len = 0, rtiles = 0, stiles = 0,
           rstiles = 0, z_words = 0;
   int S = 0, K = 0, passes = 0, digit_bits = 8;
