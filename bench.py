@@ -356,7 +356,8 @@ def serve_trace(L, n_req, slots, load):
                       "batches": st["batches"] - st0["batches"],
                       "max_inflight": st["max_inflight"]},
         "refills": st["refills"],
-        "allocations": {"warm_up": allocs_warm, "timed_closed_loops": allocs_timed},
+        "allocations": {"warm_up": allocs_warm, "timed_closed_loops": allocs_timed,
+                        "open_loop": st["allocations"] - st0["allocations"]},
     }
 
 
