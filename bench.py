@@ -28,7 +28,11 @@ import time
 # four queues two of them share one, measured 6 % of the serving capacity
 # (68,400-69,000 against 72,000-74,800 systems/s on one box).  An environment
 # that already asks for four (HIP's default) is raised to eight.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+try:
+    _hw_queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0)
+except ValueError:
+    _hw_queues = 0
+if _hw_queues < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import numpy as np
