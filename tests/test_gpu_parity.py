@@ -230,6 +230,42 @@ def test_work_split_does_not_change_results(nr, oracle, group, parts):
         lib.nl_set_autotune(was)
 
 
+@pytest.mark.parametrize("group,parts", [(1, 1), (16, 2), (64, 4)])
+def test_grid_bound_dense_and_dilute_cells(nr, oracle, group, parts):
+    # The pair-kernel grids stop at a bound on the work items: per system
+    # ceil(n / group) + min(n, bins) home groups.  A very dense periodic box
+    # (about 30 atoms per bin: several groups per bin) and a dilute periodic
+    # gas (many more bins than atoms: one group per atom) sit at either end
+    # of that bound, beside an isolated cluster; every list must match the
+    # oracle whatever the split.
+    from neighborlist_rs import workloads as W
+    from neighborlist_rs._lib import lib
+    rng = np.random.default_rng(77)
+    dense = rng.uniform(0.0, 6.0, size=(800, 3))
+    gas = rng.uniform(0.0, 40.0, size=(60, 3))
+    cluster = rng.normal(scale=4.0, size=(500, 3))
+    pos = np.vstack([dense, gas, cluster])
+    sizes = np.array([800, 60, 500], dtype=np.int64)
+    cells = [np.eye(3) * 6.0, np.eye(3) * 40.0, None]
+    cut = [3.0, 5.0]
+    g0, p0 = C.c_int(), C.c_int()
+    lib.nl_get_work_split(C.byref(g0), C.byref(p0))
+    was = lib.nl_get_autotune()
+    lib.nl_set_autotune(0)
+    lib.nl_set_work_split(group, parts)
+    try:
+        _print_case("grid bound", group, parts)
+        batch, rows = W.batch_arrays(sizes, cells)
+        ocells = [None if H is None else (H, [1, 1, 1]) for H in cells]
+        want = oracle.search_batch_multi(pos, batch, ocells, cut, False)
+        got = nr.build_neighborlists_batch_multi(pos, batch, rows, list(cut))
+        for k in range(len(cut)):
+            assert_same_edges(as_tuple(got[k]), want[k], f"G={group} P={parts} list {k}")
+    finally:
+        lib.nl_set_work_split(g0.value, p0.value)
+        lib.nl_set_autotune(was)
+
+
 @pytest.mark.parametrize("disjoint", [False, True])
 def test_twenty_cutoffs_vs_oracle(nr, oracle, disjoint):
     """More lists than one pair-kernel launch holds (8): several launches per pass,
